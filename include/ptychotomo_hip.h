@@ -1,0 +1,207 @@
+/*
+ * ptychotomo_hip.h -- C ABI of the MI355X (gfx950) ptycho-tomography library.
+ *
+ * Plain C: opaque handles, raw device pointers, sizes, int status codes.
+ * Complex arrays are interleaved complex64 (re, im float pairs), C order.
+ * Every pointer argument named as a buffer is a device pointer; the library
+ * never reads host memory through them.  `stream` is a hipStream_t (NULL =
+ * the null stream).  Calls only enqueue work, except the set-up calls
+ * (*_create, pt_tomo_set_theta, pt_ptycho_set_scan_probe), which block.
+ * A handle owns scratch memory and must not be used by two streams at once.
+ *
+ * Status codes: PT_OK (0) or a negative PT_ERR_*; pt_last_error() returns a
+ * thread-local message for the last failure.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   class radonusfft   src/include/radonusfft.cuh:3-37, src/cuda/radonusfft.i:15-46
+ *   class ptychofft    src/include/ptychofft.cuh:3-36,  src/cuda/ptychofft.i:60-94
+ *   CuPy chains of     src/ptychotomo/solver.py (solver kernels, pt_k_*)
+ */
+#ifndef PTYCHOTOMO_HIP_H
+#define PTYCHOTOMO_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_OK 0
+#define PT_ERR_ARG (-1)   /* null handle / pointer, bad enum */
+#define PT_ERR_SIZE (-2)  /* size precondition violated (n%4, radix, ...) */
+#define PT_ERR_HIP (-3)   /* HIP runtime error */
+#define PT_ERR_STATE (-4) /* e.g. operator called before set_theta */
+
+#define PT_MODEL_GAUSSIAN 0
+#define PT_MODEL_POISSON 1
+
+typedef struct pt_tomo pt_tomo;
+typedef struct pt_ptycho pt_ptycho;
+typedef void* pt_stream; /* hipStream_t */
+
+/* Thread-local text of the last error (never NULL). */
+const char* pt_last_error(void);
+/* Library ABI version (increments on incompatible change). */
+int pt_abi_version(void);
+
+/* ------------------------------------------------------------------ tomo
+ * USFFT Radon transform on a slab of pnz slices of an n x n object.
+ * f: [pnz, n, n] complex64, g: [ntheta, pnz, n] complex64.
+ */
+
+/* radonusfft::radonusfft(Ntheta, Nz, N)  radonusfft.cu:5-39.
+ * Requires n % 4 == 0 and 3n/2 factorable into radices {2,3,4,5,8}. */
+int pt_tomo_create(int ntheta, int pnz, int n, pt_tomo** out);
+/* radonusfft::~radonusfft  radonusfft.cu:41-52 */
+int pt_tomo_destroy(pt_tomo* h);
+/* radonusfft::setobj(theta)  radonusfft.cu:141-144.  theta: [ntheta] float32
+ * device array.  Also builds the sample-position tables and the tile bins of
+ * the deterministic adjoint gridding (blocking). */
+int pt_tomo_set_theta(pt_tomo* h, const float* theta, pt_stream stream);
+/* radonusfft::fwd(g, f)  radonusfft.cu:54-92 (unnormalised). */
+int pt_tomo_fwd(pt_tomo* h, void* g, const void* f, pt_stream stream);
+/* radonusfft::adj(f, g)  radonusfft.cu:94-139 (unnormalised, with the
+ * reference's edge-replicating detector pad). */
+int pt_tomo_adj(pt_tomo* h, void* f, const void* g, pt_stream stream);
+/* Same operators with the output multiplied by `scale` (solver.py:71,79
+ * multiply by coeftomo after the native call). */
+int pt_tomo_fwd_scaled(pt_tomo* h, void* g, const void* f, float scale, pt_stream stream);
+int pt_tomo_adj_scaled(pt_tomo* h, void* f, const void* g, float scale, pt_stream stream);
+
+/* ---------------------------------------------------------------- ptycho
+ * Probe x object patch, D x D far field per scan position.
+ * f: [ptheta, nz, n] complex64, g: [ptheta, nscan, detx, dety] complex64.
+ */
+
+/* ptychofft::ptychofft(Ntheta, Nz, N, Ntheta0, Nscan, detx, dety, Nprb)
+ * ptychofft.cu:5-33 (Ntheta0 is unused there and here).
+ * Requires detx == dety in {8,16,32,64} and nprb <= detx. */
+int pt_ptycho_create(int ptheta, int nz, int n, int ntheta0, int nscan, int detx, int dety,
+                     int nprb, pt_ptycho** out);
+/* ptychofft::~ptychofft  ptychofft.cu:35-46 */
+int pt_ptycho_destroy(pt_ptycho* h);
+/* ptychofft::setobj(scan, prb)  ptychofft.cu:48-56.  scan: [2, ptheta, nscan]
+ * float32 (x plane first), prb: [nprb, nprb] complex64, device arrays.
+ * Rounds positions, builds the shift factors and the tile bins of the
+ * deterministic overlap-add (blocking).  Positions whose rounded value
+ * places the probe outside the object are rejected (PT_ERR_ARG); negative
+ * rounded positions are skipped, as in the reference. */
+int pt_ptycho_set_scan_probe(pt_ptycho* h, const float* scan, const void* prb, pt_stream stream);
+/* ptychofft::fwd(g, f)  ptychofft.cu:58-76 (unnormalised). */
+int pt_ptycho_fwd(pt_ptycho* h, void* g, const void* f, pt_stream stream);
+/* ptychofft::adj(f, g)  ptychofft.cu:78-93 (unnormalised). */
+int pt_ptycho_adj(pt_ptycho* h, void* f, const void* g, pt_stream stream);
+/* Same with the output multiplied by `scale` (coefptycho, solver.py:104,124). */
+int pt_ptycho_fwd_scaled(pt_ptycho* h, void* g, const void* f, float scale, pt_stream stream);
+int pt_ptycho_adj_scaled(pt_ptycho* h, void* f, const void* g, float scale, pt_stream stream);
+
+/* Fused ptychography gradient of cg_ptycho (solver.py:219-225):
+ *   fpsi = coefptycho * F(psi)
+ *   r    = fpsi - d*fpsi/(|fpsi|^2+1e-32)          (poisson)
+ *        = fpsi - sqrt(d)*exp(i*angle(fpsi))        (gaussian)
+ *   grad = coefptycho * F*(r) - rho*(h - psi + lamd/rho)
+ * with d = coefdata*data, data: [ptheta, nscan, detx, dety] float32.
+ * The far field is never written to memory. */
+int pt_ptycho_grad(pt_ptycho* h, void* grad, const void* psi, const float* data, const void* hh,
+                   const void* lamd, double rho, float coefptycho, float coefdata, int model,
+                   pt_stream stream);
+/* Data term of the cg_ptycho functional (solver.py:208-212) at ngam step
+ * candidates psi + gamma*d, segmented by angle chunks of `chunk` angles:
+ *   sums[c*ngam + j] = sum_{chunk c} |a|^2 - 2 d mlog|a|   (poisson)
+ *                    = sum_{chunk c} (|a| - sqrt(d))^2      (gaussian)
+ *   a = coefptycho*F(psi + gammas[c*ngam+j]*dir).
+ * sums, gammas: device FP64 arrays.  dir may be NULL (then only gamma = 0
+ * is meaningful).  With lagr != 0 the poisson sum also subtracts
+ * d - 2 d mlog(sqrt d), giving lagr[0] of take_lagr (solver.py:294-296). */
+int pt_ptycho_minf(pt_ptycho* h, double* sums, const void* psi, const void* dir, const float* data,
+                   const double* gammas, int ngam, int chunk, float coefptycho, float coefdata,
+                   int model, int lagr, pt_stream stream);
+
+/* ------------------------------------------------------- solver kernels
+ * Complex64 vectors of length `len`, optionally split into `nseg` equal
+ * contiguous segments (per-chunk scalars).  Reductions accumulate in FP64
+ * and write their results to device memory.  Scalar arrays named *_dev are
+ * device FP64 arrays indexed by segment (complex scalars as re,im pairs).
+ */
+/* out[2*s..] = sum_seg conj(x)*y */
+int pt_k_dot(double* out, const void* x, const void* y, long len, int nseg, pt_stream stream);
+/* out[s] = sum_seg |x|^2 */
+int pt_k_norm2(double* out, const void* x, long len, int nseg, pt_stream stream);
+/* out[s] = sum_seg |a*x + b*y|^2 with complex host scalars a, b */
+int pt_k_norm2_lin(double* out, const void* x, const void* y, double ar, double ai, double br,
+                   double bi, long len, int nseg, pt_stream stream);
+/* z = a[s]*x + b[s]*y with complex device scalars (per segment); y may be NULL */
+int pt_k_lincomb(void* z, const double* a_dev, const void* x, const double* b_dev, const void* y,
+                 long len, int nseg, pt_stream stream);
+/* Dai-Yuan direction update (solver.py:195-196, 230-231) per segment:
+ *   d = -grad + beta[s]*d,  beta = ||grad||^2 / sum(conj(d)*(grad-grad0)),
+ * beta given as complex device scalars; first != 0 gives d = -grad. */
+int pt_k_dy_update(void* d, const void* grad, const double* beta_dev, int first, long len,
+                   int nseg, pt_stream stream);
+/* Dai-Yuan reductions per segment: out[3*s+0] = ||grad||^2,
+ * out[3*s+1], out[3*s+2] = re, im of sum conj(d)*(grad-grad0). */
+int pt_k_dy_reduce(double* out, const void* grad, const void* grad0, const void* d, long len,
+                   int nseg, pt_stream stream);
+
+/* fwd_reg (solver.py:128-134): out [3,nz,ny,nx] = J u, forward differences
+ * times 2/sqrt(3), last plane 0. */
+int pt_k_reg_fwd(void* out, const void* u, int nz, int ny, int nx, pt_stream stream);
+/* adj_reg (solver.py:137-146): out [nz,ny,nx] = J* g (times -2/sqrt(3)). */
+int pt_k_reg_adj(void* out, const void* g, int nz, int ny, int nx, pt_stream stream);
+/* solve_reg (solver.py:257-264): z = J u + mu/tau, isotropic soft threshold at
+ * alpha/tau over the 3 components. */
+int pt_k_reg_solve(void* out, const void* u, const void* mu, double tau, double alpha, int nz,
+                   int ny, int nx, pt_stream stream);
+
+/* exptomo (solver.py:59-60) with an optional extra phase (solver.py:322):
+ * out = exp(i*c*psi) * exp(i*phase). */
+int pt_k_exptomo(void* out, const void* psi, double c, double phase, long len, pt_stream stream);
+/* mlog (solver.py:48-52): log(x) with |x| < 1e-32 replaced by 1e-32. */
+int pt_k_mlog(void* out, const void* x, long len, pt_stream stream);
+
+/* takexi statistics (solver.py:151-156,164): out[0] = sum angle(psi[:,:,r:2r]),
+ * out[1] = sum angle(psi[:,:,n-2r:n-r]), out[2] = max |psi - lamd/rho|. */
+int pt_k_takexi_stats(double* out, const void* psi, const void* lamd, double rho, int ntheta,
+                      int nz, int n, int r, pt_stream stream);
+/* takexi body (solver.py:158-167): given pshift and kscale = c/max|K_raw|:
+ *   t = (psi - lamd/rho) e^{-i pshift}; K = i*kc*t*kscale;
+ *   xi0 = K * (-i mlog(t) / c1) * coeftomo. */
+int pt_k_takexi(void* xi0, void* K, const void* psi, const void* lamd, double rho, double pshift,
+                double kc, double kscale, double c1, double coeftomo, long len, pt_stream stream);
+
+/* cg_tomo pieces (solver.py:180-203), single segment:
+ * r = conj(K)*(K*Ru - xi0), out[0] = ||K*Ru - xi0||^2 */
+int pt_k_tomo_resid(void* r, double* out, const void* K, const void* Ru, const void* xi0, long len,
+                    pt_stream stream);
+/* out[0] = Re sum conj(K*Ru - xi0)*(K*Rd), out[1] = ||K*Rd||^2 */
+int pt_k_tomo_ls(double* out, const void* K, const void* Ru, const void* Rd, const void* xi0,
+                 long len, pt_stream stream);
+/* grad += tau * J*(J u - xi1) (adj_reg of fwd_reg residual, solver.py:188-190),
+ * out[0] = ||J u - xi1||^2.  u: [nz,ny,nx], xi1: [3,nz,ny,nx]. */
+int pt_k_reg_grad(void* grad, double* out, const void* u, const void* xi1, double tau, int nz,
+                  int ny, int nx, pt_stream stream);
+/* out[0] = Re sum conj(J u - xi1)*J d, out[1] = ||J d||^2 */
+int pt_k_reg_ls(double* out, const void* u, const void* d, const void* xi1, int nz, int ny, int nx,
+                pt_stream stream);
+/* cg_ptycho rho term (solver.py:213) per segment, r0 = h - psi + lamd/rho:
+ * out[3s] = ||r0||^2, out[3s+1] = Re sum conj(r0)*d, out[3s+2] = ||d||^2 */
+int pt_k_rho_terms(double* out, const void* h, const void* psi, const void* lamd, const void* d,
+                   double rho, long len, int nseg, pt_stream stream);
+/* out = sum conj(a)*(b - c) (complex, re/im) -- take_lagr terms 2 and 4 */
+int pt_k_dot_diff(double* out, const void* a, const void* b, const void* c, long len,
+                  pt_stream stream);
+/* out[0] = sum_i sqrt(sum_k |phi_k(i)|^2), phi: [3, plane] -- take_lagr term 1 */
+int pt_k_tv_norm(double* out, const void* phi, long plane, pt_stream stream);
+/* z = a*x + b*y + c*w with complex host scalars (y, w may be NULL) */
+int pt_k_lincomb3(void* z, double ar, double ai, const void* x, double br, double bi, const void* y,
+                  double cr, double ci, const void* w, long len, pt_stream stream);
+/* y += a[s]*x per segment, complex device scalars */
+int pt_k_axpy_seg(void* y, const double* a_dev, const void* x, long len, int nseg,
+                  pt_stream stream);
+/* out = |g|^2 * scale (float32), fwd_ptycho_batch intensities (solver.py:115) */
+int pt_k_intensity(float* out, const void* g, float scale, long len, pt_stream stream);
+/* out[0] = max |angle(x)| (phase-wrap check of solver.py:238-240) */
+int pt_k_max_angle(double* out, const void* x, long len, pt_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PTYCHOTOMO_HIP_H */

@@ -1,0 +1,303 @@
+"""NumPy restatement of the reference ADMM solver (TEST INFRASTRUCTURE ONLY).
+
+Line-by-line restatement of ``src/ptychotomo/solver.py`` with NumPy in place
+of CuPy and the ``oracle.usfft`` / ``oracle.ptycho`` restatements in place of
+the SWIG ``radonusfft`` / ``ptychofft`` classes.  ``dtype=np.complex128``
+gives the FP64 twin used to bound FP32 drift of the ADMM trajectory.
+
+Deviations from the reference text, each deliberate:
+* ``takexi`` uses ``r = nprb // 2``: the reference's ``prb.shape[0]/2`` is a
+  float slice bound that modern NumPy/CuPy reject (solver.py:151-155).
+* ``take_lagr`` returns the reference's 7-vector unchanged, including
+  ``lagr[6] = sum(lagr[0:5])`` which omits ``lagr[5]`` (solver.py:304).
+* ``admm`` additionally records per-iteration (rho, tau, gammas) and the
+  Lagrangian vector in ``self.history`` for trajectory comparison; the
+  iteration itself is unchanged.
+"""
+import numpy as np
+
+from . import ptycho as optycho
+from . import usfft as ousfft
+
+PLANCK_CONSTANT = 6.58211928e-19  # [keV*s]  solver.py:15
+SPEED_OF_LIGHT = 299792458e+2     # [cm/s]   solver.py:16
+
+
+class Solver:
+    """solver.py:19-336."""
+
+    def __init__(self, prb, scan, theta, det, voxelsize, energy, ntheta, nz, n,
+                 ptheta, pnz, dtype=np.complex64, workers=None):
+        self.cdt = np.dtype(dtype)
+        self.rdt = np.float32 if self.cdt == np.complex64 else np.float64
+        self.voxelsize = voxelsize
+        self.energy = energy
+        self.scan = np.asarray(scan, np.float32)
+        self.prb = np.asarray(prb, self.cdt)
+        self.theta = np.asarray(theta, np.float32)
+        self.ntheta = ntheta
+        self.nz = nz
+        self.n = n
+        self.nscan = self.scan.shape[2]
+        self.ndety = det[0]
+        self.ndetx = det[1]
+        self.nprb = self.prb.shape[0]
+        self.ptheta = ptheta
+        self.pnz = pnz
+        self.workers = workers
+        self._chunk_scan = None
+        rdt = self.rdt
+        # solver.py:43-46
+        self.coeftomo = rdt(1 / np.sqrt(self.ntheta * self.n / 2))
+        self.coefptycho = rdt(1) / np.abs(self.prb).max().astype(rdt)
+        self.coefdata = rdt(1) / (rdt(self.ndetx * self.ndety)
+                                  * (np.abs(self.prb) ** 2).max().astype(rdt))
+        self.history = []
+
+    # solver.py:48-52
+    def mlog(self, psi):
+        res = psi.copy()
+        res[np.abs(psi) < 1e-32] = 1e-32
+        return np.log(res)
+
+    def wavenumber(self):
+        return 2 * np.pi / (2 * np.pi * PLANCK_CONSTANT * SPEED_OF_LIGHT / self.energy)
+
+    def exptomo(self, psi):
+        return np.exp(1j * psi * self.voxelsize * self.wavenumber() / self.coeftomo).astype(self.cdt)
+
+    def logtomo(self, psi):
+        return (-1j / self.wavenumber() * self.mlog(psi) / self.voxelsize * self.coeftomo).astype(self.cdt)
+
+    # solver.py:67-97
+    def fwd_tomo(self, u):
+        res = ousfft.fwd(u, self.theta, self.n, self.workers, self.cdt)
+        return (res * self.coeftomo).astype(self.cdt)
+
+    def adj_tomo(self, data):
+        res = ousfft.adj(data, self.theta, self.n, self.workers, self.cdt)
+        return (res * self.coeftomo).astype(self.cdt)
+
+    def fwd_tomo_batch(self, u):
+        res = np.zeros([self.ntheta, self.nz, self.n], dtype=self.cdt)
+        for k in range(0, self.nz // self.pnz):
+            ids = np.arange(k * self.pnz, (k + 1) * self.pnz)
+            res[:, ids] = self.fwd_tomo(u[ids])
+        return res
+
+    def adj_tomo_batch(self, data):
+        res = np.zeros([self.nz, self.n, self.n], dtype=self.cdt)
+        for k in range(0, self.nz // self.pnz):
+            ids = np.arange(k * self.pnz, (k + 1) * self.pnz)
+            res[ids] = self.adj_tomo(data[:, ids])
+        return res
+
+    # ptychofft::setobj stand-in: remember the chunk's scan (solver.py:113,249,290)
+    def setobj(self, ids):
+        self._chunk_scan = self.scan[:, ids]
+
+    # solver.py:100-125
+    def fwd_ptycho(self, psi):
+        res = optycho.fwd(psi, self.prb, self._chunk_scan, self.ndety, self.ndetx, self.cdt)
+        return (res * self.coefptycho).astype(self.cdt)
+
+    def fwd_ptycho_batch(self, psi):
+        data = np.zeros([self.ntheta, self.nscan, self.ndety, self.ndetx], dtype=self.rdt)
+        for k in range(0, self.ntheta // self.ptheta):
+            ids = np.arange(k * self.ptheta, (k + 1) * self.ptheta)
+            self.setobj(ids)
+            data[ids] = np.abs(self.fwd_ptycho(psi[ids])) ** 2 / self.coefdata
+        return data
+
+    def adj_ptycho(self, data):
+        res = optycho.adj(data, self.prb, self._chunk_scan, self.nz, self.n, self.cdt)
+        return (res * self.coefptycho).astype(self.cdt)
+
+    # solver.py:128-146
+    def fwd_reg(self, u):
+        res = np.zeros([3, *u.shape], dtype=self.cdt)
+        res[0, :, :, :-1] = u[:, :, 1:] - u[:, :, :-1]
+        res[1, :, :-1, :] = u[:, 1:, :] - u[:, :-1, :]
+        res[2, :-1, :, :] = u[1:, :, :] - u[:-1, :, :]
+        res *= self.rdt(2 / np.sqrt(3))
+        return res
+
+    def adj_reg(self, gr):
+        res = np.zeros(gr.shape[1:], dtype=self.cdt)
+        res[:, :, 1:] = gr[0, :, :, 1:] - gr[0, :, :, :-1]
+        res[:, :, 0] = gr[0, :, :, 0]
+        res[:, 1:, :] += gr[1, :, 1:, :] - gr[1, :, :-1, :]
+        res[:, 0, :] += gr[1, :, 0, :]
+        res[1:, :, :] += gr[2, 1:, :, :] - gr[2, :-1, :, :]
+        res[0, :, :] += gr[2, 0, :, :]
+        res *= self.rdt(-2 / np.sqrt(3))
+        return res
+
+    # solver.py:149-168 (r = nprb // 2, see module docstring)
+    def takexi(self, psi, phi, lamd, mu, rho, tau):
+        r = self.nprb // 2
+        m1 = np.mean(np.angle(psi[:, :, r:2 * r]))
+        m2 = np.mean(np.angle(psi[:, :, psi.shape[2] - 2 * r:psi.shape[2] - r]))
+        pshift = (m1 + m2) / 2
+        t = psi - lamd / rho
+        t = (t * np.exp(-1j * pshift)).astype(self.cdt)
+        logt = self.mlog(t)
+        K = (1j * self.voxelsize * self.wavenumber() * t / self.coeftomo).astype(self.cdt)
+        K = (K / np.amax(np.abs(K))).astype(self.cdt)
+        xi0 = (K * (-1j * logt / (self.voxelsize * self.wavenumber())) * self.coeftomo).astype(self.cdt)
+        xi1 = phi - mu / tau
+        return xi0, xi1, K, pshift
+
+    # solver.py:171-177
+    def line_search(self, minf, gamma, u, fu, d, fd):
+        while minf(u, fu) - minf(u + gamma * d, fu + gamma * fd) < 0 and gamma > 1e-12:
+            gamma *= 0.5
+        if gamma <= 1e-12:
+            gamma = 0
+        return gamma
+
+    def _norm2(self, x):
+        return np.linalg.norm(x) ** 2
+
+    # solver.py:180-203
+    def cg_tomo(self, xi0, xi1, K, init, rho, tau, titer):
+        def minf(KRu, gu):
+            return rho * self._norm2(KRu - xi0) + tau * self._norm2(gu - xi1)
+        u = init.copy()
+        gamma = 2
+        gammas = []
+        for i in range(titer):
+            KRu = K * self.fwd_tomo_batch(u)
+            gu = self.fwd_reg(u)
+            grad = rho * self.adj_tomo_batch(np.conj(K) * (KRu - xi0)) + \
+                tau * self.adj_reg(gu - xi1)
+            if i == 0:
+                d = -grad
+            else:
+                d = -grad + self._norm2(grad) / np.sum(np.conj(d) * (grad - grad0)) * d
+            grad0 = grad
+            gamma = self.line_search(minf, gamma, KRu, gu, K * self.fwd_tomo_batch(d), self.fwd_reg(d))
+            gammas.append(gamma)
+            u = (u + gamma * d).astype(self.cdt)
+        self._last_tomo_gammas = gammas
+        return u
+
+    # solver.py:206-242
+    def cg_ptycho(self, data, init, h, lamd, rho, piter, model):
+        def minf(psi, fpsi):
+            if model == 'gaussian':
+                f = self._norm2(np.abs(fpsi) - np.sqrt(data))
+            elif model == 'poisson':
+                f = np.sum(np.abs(fpsi) ** 2 - 2 * data * self.mlog(np.abs(fpsi)))
+            f += rho * self._norm2(h - psi + lamd / rho)
+            return f
+        psi = init.copy()
+        gamma = 2
+        gammas = []
+        for i in range(piter):
+            fpsi = self.fwd_ptycho(psi)
+            if model == 'gaussian':
+                grad = self.adj_ptycho(fpsi - np.sqrt(data) * np.exp(1j * np.angle(fpsi)))
+            elif model == 'poisson':
+                grad = self.adj_ptycho(fpsi - data * fpsi / (np.abs(fpsi) ** 2 + 1e-32))
+            grad -= (rho * (h - psi + lamd / rho)).astype(self.cdt)
+            if i == 0:
+                d = -grad
+            else:
+                d = -grad + self._norm2(grad) / np.sum(np.conj(d) * (grad - grad0)) * d
+            grad0 = grad
+            fd = self.fwd_ptycho(d)
+            gamma = self.line_search(minf, gamma, psi, fpsi, d, fd)
+            gammas.append(gamma)
+            psi = (psi + gamma * d).astype(self.cdt)
+        if np.amax(np.abs(np.angle(psi))) > 3.14:
+            print('possible phase wrap, max computed angle', np.amax(np.abs(np.angle(psi))))
+        self._chunk_gammas.append(gammas)
+        return psi
+
+    # solver.py:245-254
+    def cg_ptycho_batch(self, data, init, h, lamd, rho, piter, model):
+        psi = init.copy()
+        self._chunk_gammas = []
+        for k in range(0, self.ntheta // self.ptheta):
+            ids = np.arange(k * self.ptheta, (k + 1) * self.ptheta)
+            self.setobj(ids)
+            datap = np.asarray(data[ids], self.rdt) * self.coefdata
+            psi[ids] = self.cg_ptycho(datap, psi[ids], h[ids], lamd[ids], rho, piter, model)
+        return psi
+
+    # solver.py:257-264
+    def solve_reg(self, u, mu, tau, alpha):
+        z = self.fwd_reg(u) + mu / tau
+        za = np.sqrt(np.real(np.sum(z * np.conj(z), 0)))
+        z[:, za <= alpha / tau] = 0
+        z[:, za > alpha / tau] -= alpha / tau * z[:, za > alpha / tau] / (za[za > alpha / tau])
+        return z.astype(self.cdt)
+
+    # solver.py:267-282
+    def update_penalty(self, psi, h, h0, phi, e, e0, rho, tau):
+        r = self._norm2(psi - h)
+        s = self._norm2(rho * (h - h0))
+        if r > 10 * s:
+            rho *= 2
+        elif s > 10 * r:
+            rho *= 0.5
+        r = self._norm2(phi - e)
+        s = self._norm2(tau * (e - e0))
+        if r > 10 * s:
+            tau *= 2
+        elif s > 10 * r:
+            tau *= 0.5
+        return rho, tau
+
+    # solver.py:285-305
+    def take_lagr(self, psi, phi, data, h, e, lamd, mu, alpha, rho, tau, model):
+        lagr = np.zeros(7, dtype=self.rdt)
+        for k in range(0, self.ntheta // self.ptheta):
+            ids = np.arange(k * self.ptheta, (k + 1) * self.ptheta)
+            self.setobj(ids)
+            fpsi = self.fwd_ptycho(psi[ids])
+            datap = np.asarray(data[ids], self.rdt) * self.coefdata
+            if model == 'poisson':
+                lagr[0] += np.sum(np.abs(fpsi) ** 2 - 2 * datap * self.mlog(np.abs(fpsi))
+                                  - (datap - 2 * datap * self.mlog(np.sqrt(datap))))
+            if model == 'gaussian':
+                lagr[0] += self._norm2(np.abs(fpsi) - np.sqrt(datap))
+        lagr[1] = alpha * np.sum(np.sqrt(np.real(np.sum(phi * np.conj(phi), 0))))
+        lagr[2] = 2 * np.sum(np.real(np.conj(lamd) * (h - psi)))
+        lagr[3] = rho * self._norm2(h - psi)
+        lagr[4] = 2 * np.sum(np.real(np.conj(mu) * (e - phi)))
+        lagr[5] = tau * self._norm2(e - phi)
+        lagr[6] = np.sum(lagr[0:5])
+        return lagr
+
+    # solver.py:308-336
+    def admm(self, data, h, e, psi, phi, lamd, mu, u, alpha, piter, titer, niter,
+             model, lagr_every=10, verbose=False):
+        rho = 0.5
+        tau = 0.5
+        self.history = []
+        for m in range(niter):
+            h0, e0 = h, e
+            psi = self.cg_ptycho_batch(data, psi, h, lamd, rho, piter + (m < 2) * 32, model)
+            xi0, xi1, K, pshift = self.takexi(psi, phi, lamd, mu, rho, tau)
+            u = self.cg_tomo(xi0, xi1, K, u, rho, tau, titer)
+            phi = self.solve_reg(u, mu, tau, alpha)
+            h = (self.exptomo(self.fwd_tomo_batch(u)) * np.exp(1j * pshift)).astype(self.cdt)
+            e = self.fwd_reg(u)
+            lamd = (lamd + rho * (h - psi)).astype(self.cdt)
+            mu = (mu + tau * (e - phi)).astype(self.cdt)
+            rec = {"m": m, "rho_used": rho, "tau_used": tau, "pshift": float(pshift),
+                   "ptycho_gammas": self._chunk_gammas, "tomo_gammas": self._last_tomo_gammas}
+            rho, tau = self.update_penalty(psi, h, h0, phi, e, e0, rho, tau)
+            rec["rho"] = rho
+            rec["tau"] = tau
+            if lagr_every and np.mod(m, lagr_every) == 0:
+                lagr = self.take_lagr(psi, phi, data, h, e, lamd, mu, alpha, rho, tau, model)
+                rec["lagr"] = [float(v) for v in lagr]
+                if verbose:
+                    print("%d/%d) rho=%.2e, tau=%.2e, Lagrangian terms:  %.2e %.2e %.2e %.2e %.2e %.2e, Sum: %.2e" %
+                          (m, niter, rho, tau, *lagr))
+            self.history.append(rec)
+        self.state = dict(h=h, e=e, psi=psi, phi=phi, lamd=lamd, mu=mu, u=u, rho=rho, tau=tau)
+        return u, psi

@@ -1,0 +1,109 @@
+"""ctypes binding of ``libptychotomo_hip.so`` (the C ABI in
+``include/ptychotomo_hip.h``).
+
+The shared library is required: importing this module without it raises
+``ImportError`` -- there is no CPU fallback anywhere in the package.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("PTYCHOTOMO_HIP_LIB", os.path.join(HERE, "libptychotomo_hip.so"))
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"ptychotomo_amd: HIP library not found at {LIB_PATH}; build it with "
+        "`python -m ptychotomo_amd.build`")
+
+lib = ctypes.CDLL(LIB_PATH)
+
+c_int = ctypes.c_int
+c_long = ctypes.c_long
+c_float = ctypes.c_float
+c_double = ctypes.c_double
+vp = ctypes.c_void_p
+PT_OK = 0
+PT_MODEL = {"gaussian": 0, "poisson": 1}
+
+_SIGS = {
+    "pt_last_error": (ctypes.c_char_p, []),
+    "pt_abi_version": (c_int, []),
+    "pt_tomo_create": (c_int, [c_int, c_int, c_int, ctypes.POINTER(vp)]),
+    "pt_tomo_destroy": (c_int, [vp]),
+    "pt_tomo_set_theta": (c_int, [vp, vp, vp]),
+    "pt_tomo_fwd": (c_int, [vp, vp, vp, vp]),
+    "pt_tomo_adj": (c_int, [vp, vp, vp, vp]),
+    "pt_tomo_fwd_scaled": (c_int, [vp, vp, vp, c_float, vp]),
+    "pt_tomo_adj_scaled": (c_int, [vp, vp, vp, c_float, vp]),
+    "pt_ptycho_create": (c_int, [c_int] * 8 + [ctypes.POINTER(vp)]),
+    "pt_ptycho_destroy": (c_int, [vp]),
+    "pt_ptycho_set_scan_probe": (c_int, [vp, vp, vp, vp]),
+    "pt_ptycho_fwd": (c_int, [vp, vp, vp, vp]),
+    "pt_ptycho_adj": (c_int, [vp, vp, vp, vp]),
+    "pt_ptycho_fwd_scaled": (c_int, [vp, vp, vp, c_float, vp]),
+    "pt_ptycho_adj_scaled": (c_int, [vp, vp, vp, c_float, vp]),
+    "pt_ptycho_grad": (c_int, [vp, vp, vp, vp, vp, vp, c_double, c_float, c_float, c_int, vp]),
+    "pt_ptycho_minf": (c_int, [vp, vp, vp, vp, vp, vp, c_int, c_int, c_float, c_float, c_int,
+                               c_int, vp]),
+    "pt_k_dot": (c_int, [vp, vp, vp, c_long, c_int, vp]),
+    "pt_k_norm2": (c_int, [vp, vp, c_long, c_int, vp]),
+    "pt_k_norm2_lin": (c_int, [vp, vp, vp, c_double, c_double, c_double, c_double, c_long, c_int,
+                               vp]),
+    "pt_k_lincomb": (c_int, [vp, vp, vp, vp, vp, c_long, c_int, vp]),
+    "pt_k_dy_update": (c_int, [vp, vp, vp, c_int, c_long, c_int, vp]),
+    "pt_k_dy_reduce": (c_int, [vp, vp, vp, vp, c_long, c_int, vp]),
+    "pt_k_reg_fwd": (c_int, [vp, vp, c_int, c_int, c_int, vp]),
+    "pt_k_reg_adj": (c_int, [vp, vp, c_int, c_int, c_int, vp]),
+    "pt_k_reg_solve": (c_int, [vp, vp, vp, c_double, c_double, c_int, c_int, c_int, vp]),
+    "pt_k_exptomo": (c_int, [vp, vp, c_double, c_double, c_long, vp]),
+    "pt_k_mlog": (c_int, [vp, vp, c_long, vp]),
+    "pt_k_takexi_stats": (c_int, [vp, vp, vp, c_double, c_int, c_int, c_int, c_int, vp]),
+    "pt_k_takexi": (c_int, [vp, vp, vp, vp, c_double, c_double, c_double, c_double, c_double,
+                            c_double, c_long, vp]),
+    "pt_k_tomo_resid": (c_int, [vp, vp, vp, vp, vp, c_long, vp]),
+    "pt_k_tomo_ls": (c_int, [vp, vp, vp, vp, vp, c_long, vp]),
+    "pt_k_reg_grad": (c_int, [vp, vp, vp, vp, c_double, c_int, c_int, c_int, vp]),
+    "pt_k_reg_ls": (c_int, [vp, vp, vp, vp, c_int, c_int, c_int, vp]),
+    "pt_k_rho_terms": (c_int, [vp, vp, vp, vp, vp, c_double, c_long, c_int, vp]),
+    "pt_k_dot_diff": (c_int, [vp, vp, vp, vp, c_long, vp]),
+    "pt_k_tv_norm": (c_int, [vp, vp, c_long, vp]),
+    "pt_k_lincomb3": (c_int, [vp, c_double, c_double, vp, c_double, c_double, vp, c_double,
+                              c_double, vp, c_long, vp]),
+    "pt_k_axpy_seg": (c_int, [vp, vp, vp, c_long, c_int, vp]),
+    "pt_k_max_angle": (c_int, [vp, vp, c_long, vp]),
+    "pt_k_intensity": (c_int, [vp, vp, c_float, c_long, vp]),
+}
+
+EXPORTED = sorted(_SIGS)
+
+
+def _bind():
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+
+
+_bind()
+
+
+class PtError(RuntimeError):
+    pass
+
+
+def check(rc, what=""):
+    if rc != PT_OK:
+        msg = lib.pt_last_error().decode(errors="replace")
+        raise PtError(f"{what or 'ptychotomo_hip'} failed ({rc}): {msg}")
+    return rc
+
+
+def call(name, *args):
+    """Call a C-ABI function and raise PtError on a non-zero status."""
+    return check(getattr(lib, name)(*args), name)
+
+
+def missing_symbols():
+    return [n for n in EXPORTED if getattr(lib, n, None) is None]

@@ -1,0 +1,80 @@
+"""Build the HIP shared library ``libptychotomo_hip.so`` for gfx950.
+
+Usage: ``python -m ptychotomo_amd.build`` (or ``build()`` from Python).
+Compiles every ``csrc/*.hip`` with hipcc into one shared object placed next
+to this file.  Packed FP32 VALU instructions are disabled for all device
+code.
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libptychotomo_hip.so")
+ARCH = os.environ.get("PT_OFFLOAD_ARCH", "gfx950")
+
+SOURCES = ["vec.hip", "ptycho.hip", "tomo.hip"]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def flags():
+    return [
+        "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-Wno-unused-command-line-argument", "-Wno-unused-result",
+        "-I", os.path.join(HERE, "..", "include"),
+    ]
+
+
+def build(verbose=False, jobs=4):
+    cc = hipcc()
+    tmp = tempfile.mkdtemp(prefix="pt_build_")
+    objs = []
+    procs = []
+    try:
+        for src in SOURCES:
+            obj = os.path.join(tmp, src.replace(".hip", ".o"))
+            cmd = [cc, *flags(), "-c", os.path.join(CSRC, src), "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                stderr=subprocess.STDOUT, text=True)))
+            objs.append(obj)
+            if len(procs) >= jobs:
+                _wait(procs.pop(0))
+        while procs:
+            _wait(procs.pop(0))
+        out_tmp = LIB + ".tmp"
+        cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out_tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode:
+            raise RuntimeError("link failed:\n" + r.stdout)
+        os.replace(out_tmp, LIB)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return LIB
+
+
+def _wait(item):
+    src, p = item
+    out, _ = p.communicate()
+    if p.returncode:
+        raise RuntimeError(f"hipcc failed on {src}:\n{out}")
+    lines = [l for l in out.splitlines() if "packed-fp32-ops" not in l and l.strip()]
+    if lines:
+        sys.stderr.write(f"[{src}]\n" + "\n".join(lines[-40:]) + "\n")
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

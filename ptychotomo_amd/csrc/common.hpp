@@ -1,0 +1,372 @@
+// Shared device/host helpers for the ptychotomo HIP library (gfx950).
+//
+//  * status codes and a per-thread last-error string for the C ABI
+//  * complex helpers on float2
+//  * compile-time trig (twiddle constants) and an in-register FFT whose
+//    length is a template parameter (detector frames: 8..64 points per lane)
+//  * an LDS Stockham FFT with a run-time mixed-radix plan (2,3,4,5,8) for the
+//    tomography lines (N = 3n/2 points)
+//  * FP64 wavefront / workgroup reductions
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <utility>
+
+#include "../../include/ptychotomo_hip.h"
+
+namespace pt {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+
+#define PT_HIP(call)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ::pt::set_error(std::string(#call) + ": " + hipGetErrorString(e_));     \
+      return PT_ERR_HIP;                                                      \
+    }                                                                         \
+  } while (0)
+
+#define PT_REQUIRE(cond, code, msg)                                           \
+  do {                                                                        \
+    if (!(cond)) {                                                            \
+      ::pt::set_error(msg);                                                   \
+      return code;                                                            \
+    }                                                                         \
+  } while (0)
+
+#define PT_LAUNCH_CHECK()                                                     \
+  do {                                                                        \
+    hipError_t e_ = hipGetLastError();                                        \
+    if (e_ != hipSuccess) {                                                   \
+      ::pt::set_error(std::string("kernel launch: ") + hipGetErrorString(e_)); \
+      return PT_ERR_HIP;                                                      \
+    }                                                                         \
+  } while (0)
+
+// ---------------------------------------------------------------- complex
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by SIGN*i  (SIGN = -1: -i, SIGN = +1: +i)
+template <int SIGN>
+__host__ __device__ inline float2 mul_i(float2 a) {
+  return SIGN < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+// ---------------------------------------------------------------- compile-time helpers
+template <int I>
+using ic = std::integral_constant<int, I>;
+
+template <int B, int E, typename F>
+__host__ __device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(ic<B>{});
+    static_for<B + 1, E>(static_cast<F&&>(f));
+  }
+}
+
+constexpr double kPi = 3.14159265358979323846;
+
+// sin/cos of 2*pi*num/den evaluated exactly enough at compile time (Taylor
+// series after octant reduction; |error| < 1e-16).
+constexpr double ct_sin_small(double x) {
+  double term = x, sum = x;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x * x / ((2 * k) * (2 * k + 1));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double ct_cos_small(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x * x / ((2 * k - 1) * (2 * k));
+    sum += term;
+  }
+  return sum;
+}
+// cos/sin of 2*pi*m/L for integer m, L (m reduced mod L)
+constexpr double ct_cos2pi(long m, long L) {
+  long r = ((m % L) + L) % L;
+  // reduce to first quadrant using symmetry on 8*r/L
+  double x = 2.0 * kPi * (double)r / (double)L;
+  if (x > kPi) x -= 2.0 * kPi;                 // (-pi, pi]
+  double ax = x < 0 ? -x : x;                  // [0, pi]
+  double c;
+  if (ax <= kPi / 4) c = ct_cos_small(ax);
+  else if (ax <= 3 * kPi / 4) c = ct_sin_small(kPi / 2 - ax);
+  else c = -ct_cos_small(kPi - ax);
+  return c;
+}
+constexpr double ct_sin2pi(long m, long L) {
+  long r = ((m % L) + L) % L;
+  double x = 2.0 * kPi * (double)r / (double)L;
+  if (x > kPi) x -= 2.0 * kPi;
+  double ax = x < 0 ? -x : x;
+  double s;
+  if (ax <= kPi / 4) s = ct_sin_small(ax);
+  else if (ax <= 3 * kPi / 4) s = ct_cos_small(kPi / 2 - ax);
+  else s = ct_sin_small(kPi - ax);
+  return x < 0 ? -s : s;
+}
+
+// W_L^(SIGN*m) = exp(SIGN * 2 pi i m / L)
+template <int L, int M, int SIGN>
+struct Tw {
+  static constexpr long mm = ((long)M % L + L) % L;
+  static constexpr float re = (float)ct_cos2pi(mm, L);
+  static constexpr float im = (float)(SIGN * ct_sin2pi(mm, L));
+  static constexpr bool one = (mm == 0);
+  static constexpr bool minus_one = (2 * mm == L);
+  static constexpr bool plus_i = (4 * mm == L && SIGN > 0) || (4 * mm == 3 * L && SIGN < 0);
+  static constexpr bool minus_i = (4 * mm == L && SIGN < 0) || (4 * mm == 3 * L && SIGN > 0);
+};
+
+template <int L, int M, int SIGN>
+__host__ __device__ __forceinline__ float2 twiddle_ct(float2 a) {
+  using T = Tw<L, M, SIGN>;
+  if constexpr (T::one) return a;
+  else if constexpr (T::minus_one) return make_float2(-a.x, -a.y);
+  else if constexpr (T::plus_i) return make_float2(-a.y, a.x);
+  else if constexpr (T::minus_i) return make_float2(a.y, -a.x);
+  else return make_float2(a.x * T::re - a.y * T::im, a.x * T::im + a.y * T::re);
+}
+
+// ---------------------------------------------------------------- small DFTs (compile-time)
+template <int SIGN>
+__host__ __device__ __forceinline__ void dft2(float2& a0, float2& a1) {
+  float2 t = a0;
+  a0 = cadd(t, a1);
+  a1 = csub(t, a1);
+}
+
+template <int SIGN>
+__host__ __device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  float2 t2 = cadd(a1, a3), t3 = mul_i<SIGN>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+template <int SIGN>
+__host__ __device__ __forceinline__ void dft8(float2* a) {
+  float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4<SIGN>(e0, e1, e2, e3);
+  dft4<SIGN>(o0, o1, o2, o3);
+  o1 = twiddle_ct<8, 1, SIGN>(o1);
+  o2 = twiddle_ct<8, 2, SIGN>(o2);
+  o3 = twiddle_ct<8, 3, SIGN>(o3);
+  a[0] = cadd(e0, o0); a[4] = csub(e0, o0);
+  a[1] = cadd(e1, o1); a[5] = csub(e1, o1);
+  a[2] = cadd(e2, o2); a[6] = csub(e2, o2);
+  a[3] = cadd(e3, o3); a[7] = csub(e3, o3);
+}
+
+template <int R, int SIGN>
+__host__ __device__ __forceinline__ void dft_ct(float2* a) {
+  if constexpr (R == 2) dft2<SIGN>(a[0], a[1]);
+  else if constexpr (R == 4) dft4<SIGN>(a[0], a[1], a[2], a[3]);
+  else if constexpr (R == 8) dft8<SIGN>(a);
+  else static_assert(R == 2, "radix");
+}
+
+// ---------------------------------------------------------------- in-register FFT
+// Unnormalised DFT of L points held by one lane, natural order in and out:
+//   X[k] = sum_j x[j] exp(SIGN * 2 pi i j k / L).
+// Stockham passes with compile-time radices; all indices are constants so
+// the data stay in VGPRs.
+template <int L>
+struct RegPlan {
+  // first radix used at stage with Ns already done
+  static constexpr int radix(int rem) { return rem == 16 ? 4 : (rem % 8 == 0 ? 8 : (rem % 4 == 0 ? 4 : 2)); }
+};
+
+template <int L, int Ns, int SIGN>
+__host__ __device__ __forceinline__ void reg_fft_pass(float2 (&v)[L]) {
+  if constexpr (Ns < L) {
+    constexpr int R = RegPlan<L>::radix(L / Ns);
+    float2 out[L];
+    static_for<0, L / R>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      constexpr int k = j % Ns;
+      float2 a[R];
+      static_for<0, R>([&](auto icc) {
+        constexpr int i = decltype(icc)::value;
+        a[i] = twiddle_ct<Ns * R, i * k, SIGN>(v[j + i * (L / R)]);
+      });
+      dft_ct<R, SIGN>(a);
+      constexpr int idxD = (j / Ns) * Ns * R + k;
+      static_for<0, R>([&](auto icc) {
+        constexpr int i = decltype(icc)::value;
+        out[idxD + i * Ns] = a[i];
+      });
+    });
+    static_for<0, L>([&](auto icc) { v[decltype(icc)::value] = out[decltype(icc)::value]; });
+    reg_fft_pass<L, Ns * R, SIGN>(v);
+  }
+}
+
+template <int L, int SIGN>
+__host__ __device__ __forceinline__ void reg_fft(float2 (&v)[L]) {
+  if constexpr (L > 1) reg_fft_pass<L, 1, SIGN>(v);
+}
+
+// ---------------------------------------------------------------- LDS mixed-radix FFT
+// Run-time plan for one line length N = prod(radix[i]).  Twiddle table
+// tw[m] = exp(-2 pi i m / N) lives in global memory (FP64-rounded).
+struct LinePlan {
+  int N;
+  int npass;
+  int radix[12];
+};
+
+#define PT_LDS_FFT_MAXV 24
+
+// host: true if every pass of `p` fits the per-thread register budget
+inline bool lds_fft_fits(const LinePlan& p, int nlines, int nthr) {
+  for (int i = 0; i < p.npass; ++i) {
+    int R = p.radix[i];
+    long nb = (long)nlines * (p.N / R);
+    long per = (nb + nthr - 1) / nthr;
+    if (per * R > PT_LDS_FFT_MAXV) return false;
+  }
+  return true;
+}
+
+__device__ __forceinline__ void dft_rt(float2* a, int R, int sign) {
+  // generic small DFTs for run-time radix (sign = -1 forward, +1 inverse)
+  if (R == 2) {
+    dft2<1>(a[0], a[1]);
+  } else if (R == 4) {
+    if (sign < 0) dft4<-1>(a[0], a[1], a[2], a[3]);
+    else dft4<1>(a[0], a[1], a[2], a[3]);
+  } else if (R == 8) {
+    if (sign < 0) dft8<-1>(a);
+    else dft8<1>(a);
+  } else if (R == 3) {
+    const float s3 = 0.86602540378443864676f * (float)sign;
+    float2 t1 = cadd(a[1], a[2]);
+    float2 t2 = make_float2(a[0].x - 0.5f * t1.x, a[0].y - 0.5f * t1.y);
+    float2 d = csub(a[1], a[2]);
+    float2 t3 = make_float2(-s3 * d.y, s3 * d.x);  // sign*i*sqrt(3)/2*(a1-a2)
+    a[0] = cadd(a[0], t1);
+    a[1] = cadd(t2, t3);
+    a[2] = csub(t2, t3);
+  } else if (R == 5) {
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    const float s1 = 0.95105651629515357212f * (float)sign, s2 = 0.58778525229247312917f * (float)sign;
+    float2 b1 = cadd(a[1], a[4]), b4 = csub(a[1], a[4]);
+    float2 b2 = cadd(a[2], a[3]), b3 = csub(a[2], a[3]);
+    float2 x0 = a[0];
+    float2 r1 = make_float2(x0.x + c1 * b1.x + c2 * b2.x, x0.y + c1 * b1.y + c2 * b2.y);
+    float2 r2 = make_float2(x0.x + c2 * b1.x + c1 * b2.x, x0.y + c2 * b1.y + c1 * b2.y);
+    // i*(s1*b4 + s2*b3), i*(s2*b4 - s1*b3)
+    float2 q1 = make_float2(-(s1 * b4.y + s2 * b3.y), s1 * b4.x + s2 * b3.x);
+    float2 q2 = make_float2(-(s2 * b4.y - s1 * b3.y), s2 * b4.x - s1 * b3.x);
+    a[0] = make_float2(x0.x + b1.x + b2.x, x0.y + b1.y + b2.y);
+    a[1] = cadd(r1, q1);
+    a[4] = csub(r1, q1);
+    a[2] = cadd(r2, q2);
+    a[3] = csub(r2, q2);
+  }
+}
+
+// In-place FFT of `nlines` lines of length plan.N stored in LDS at
+// buf + line*stride; executed by `nthr` threads (thread index t) that all call
+// it together; contains __syncthreads().  sign = -1 forward, +1 inverse
+// (unnormalised).  The caller guarantees ceil(nlines*N/R/nthr)*R <= PT_LDS_FFT_MAXV
+// for every pass radix R (checked on the host by lds_fft_fits()).
+__device__ inline void lds_fft_lines(float2* buf, int stride, int nlines, const LinePlan& plan,
+                                     const float2* __restrict__ tw, int sign, int t, int nthr) {
+  const int N = plan.N;
+  int Ns = 1;
+  for (int p = 0; p < plan.npass; ++p) {
+    const int R = plan.radix[p];
+    const int nb = N / R;               // butterflies per line
+    const int total = nb * nlines;
+    float2 v[PT_LDS_FFT_MAXV];
+    int cnt = 0;
+    // load + twiddle + butterfly
+    for (int b = t; b < total && cnt + R <= PT_LDS_FFT_MAXV; b += nthr) {
+      int line = b / nb, j = b - line * nb;
+      int k = j % Ns;
+      float2* src = buf + line * stride;
+      float2 a[8];
+      const int step = N / (Ns * R);
+      for (int i = 0; i < R; ++i) {
+        float2 x = src[j + i * nb];
+        if (i && k) {
+          int m = (i * k * step) % N;
+          float2 w = tw[m];
+          if (sign > 0) w.y = -w.y;
+          x = cmul(x, w);
+        }
+        a[i] = x;
+      }
+      dft_rt(a, R, sign);
+      for (int i = 0; i < R; ++i) v[cnt + i] = a[i];
+      cnt += R;
+    }
+    __syncthreads();
+    cnt = 0;
+    for (int b = t; b < total && cnt + R <= PT_LDS_FFT_MAXV; b += nthr) {
+      int line = b / nb, j = b - line * nb;
+      int k = j % Ns;
+      float2* dst = buf + line * stride;
+      int idxD = (j / Ns) * Ns * R + k;
+      for (int i = 0; i < R; ++i) dst[idxD + i * Ns] = v[cnt + i];
+      cnt += R;
+    }
+    __syncthreads();
+    Ns *= R;
+  }
+}
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Sum of `NV` doubles over the workgroup; result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* smem /* >= NV*16 */) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  __syncthreads();
+  if (lane == 0)
+    for (int i = 0; i < NV; ++i) smem[i * 16 + wid] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < NV; ++i) {
+      double s = 0;
+      for (int w = 0; w < nw; ++w) s += smem[i * 16 + w];
+      v[i] = s;
+    }
+  }
+}
+
+// host: factor N into the supported radices; returns false if impossible
+bool make_line_plan(int N, LinePlan* p);
+
+}  // namespace pt

@@ -1,0 +1,649 @@
+// Ptychography operator for gfx950: probe x object patch, D x D far field.
+//
+// Replaces class ptychofft (src/cuda/ptychofft.cu, src/include/kernels.cuh
+// kernels mul/mula/takeshifts/shifts/shiftsa) and fuses the CuPy chains of
+// cg_ptycho (src/ptychotomo/solver.py:206-242).
+//
+// Layout: one 64-lane wavefront works on G = 64/D patches at a time; lane
+// (slot = lane/D, r = lane%D) holds one frame row (D complex values in
+// VGPRs).  The 2D FFT is an in-register row FFT, an LDS transpose (row stride
+// D+1 to avoid bank conflicts), and an in-register column FFT, after which
+// lane c holds far-field column c.  The frame is transposed relative to the
+// object (kernels.cuh:251-263): frame row ix + ox holds object column
+// stx + ix, frame column iy + oy holds object row sty + iy.
+//
+// Overlap-add (adjoint and fused gradient) is deterministic and free of
+// global atomics: patches are binned by rounded position into B x B tiles
+// per angle at set_scan_probe time; one wavefront owns a tile and
+// accumulates its patches, in bin order, into an LDS tile extended by P-1
+// (LDS adds issued by a single wavefront); the extended tiles go to scratch
+// and a fold kernel sums the (at most 4) tiles covering each pixel in a fixed
+// order.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+struct pt_ptycho {
+  int pt, nz, n, S, D, P;
+  int ox, oy;
+  float c;  // 1/sqrt(detx*dety)
+  int B, EB, ntx, nty;
+  long ntiles, nslots;
+  float2* prb = nullptr;        // [P*P]
+  int2* pos = nullptr;          // [pt*S] rounded (x, y); x < 0 -> skipped
+  float2* shift = nullptr;      // [pt*S] shiftx*shifty
+  int* tile_start = nullptr;    // [ntiles+1]
+  int* tile_patch = nullptr;    // [nvalid]
+  int* tile_slot = nullptr;     // [ntiles]
+  float2* scratch = nullptr;    // [nslots*EB*EB]
+  size_t scratch_cap = 0;       // elements
+  double* partial = nullptr;    // minf partial sums
+  size_t partial_cap = 0;       // doubles
+  bool ready = false;
+};
+
+namespace pt {
+namespace {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
+
+// ------------------------------------------------------------------ transforms
+// Forward 2D transform of the frame rows held by the wave (v = row r of the
+// lane's patch) -> lane c holds far-field column c (in v).
+template <int D>
+__device__ __forceinline__ void frame_fwd(float2 (&v)[D], float2* T, int slot, int r) {
+  reg_fft<D, -1>(v);
+  float2* Ts = T + slot * D * (D + 1);
+#pragma unroll
+  for (int k = 0; k < D; ++k) Ts[r * (D + 1) + k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < D; ++k) v[k] = Ts[k * (D + 1) + r];
+  __syncthreads();
+  reg_fft<D, -1>(v);
+}
+
+// Inverse of frame_fwd's layout: lane c holds column c -> lane r holds row r.
+template <int D>
+__device__ __forceinline__ void frame_inv(float2 (&v)[D], float2* T, int slot, int r) {
+  reg_fft<D, 1>(v);
+  float2* Ts = T + slot * D * (D + 1);
+#pragma unroll
+  for (int k = 0; k < D; ++k) Ts[k * (D + 1) + r] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < D; ++k) v[k] = Ts[r * (D + 1) + k];
+  __syncthreads();
+  reg_fft<D, 1>(v);
+}
+
+// Load frame row r of patch p: c * prb[ix][iy] * psi[t][sty+iy][stx+ix]
+template <int D>
+__device__ __forceinline__ void load_frame_row(float2 (&v)[D], const float2* __restrict__ f,
+                                               const float2* __restrict__ prb, int2 ps, int t, int r,
+                                               int nz, int n, int P, int ox, int oy, float cs,
+                                               bool active) {
+  const int ix = r - ox;
+  const bool rowin = active && ps.x >= 0 && ix >= 0 && ix < P;
+  const float2* fp = f + ((long)t * nz + ps.y - oy) * n + ps.x + ix;
+  const float2* pp = prb + (long)ix * P - oy;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const int iy = k - oy;
+    float2 val = make_float2(0.f, 0.f);
+    if (rowin && iy >= 0 && iy < P) {
+      float2 a = fp[(long)k * n];
+      float2 b = pp[k];
+      val = cscale(cmul(b, a), cs);
+    }
+    v[k] = val;
+  }
+}
+
+// ------------------------------------------------------------------ fwd
+template <int D>
+__global__ __launch_bounds__(WAVE) void k_fwd(float2* __restrict__ g, const float2* __restrict__ f,
+                                              const float2* __restrict__ prb,
+                                              const int2* __restrict__ pos,
+                                              const float2* __restrict__ shift, int pt, int nz, int n,
+                                              int S, int P, int ox, int oy, float cs) {
+  constexpr int G = WAVE / D;
+  __shared__ float2 T[G * D * (D + 1)];
+  const int lane = lane_id(), slot = lane / D, r = lane % D;
+  const long npatch = (long)pt * S;
+  for (long grp = blockIdx.x; grp * G < npatch; grp += gridDim.x) {
+    const long p = grp * G + slot;
+    const bool active = p < npatch;
+    int2 ps = active ? pos[p] : make_int2(-1, -1);
+    const int t = active ? (int)(p / S) : 0;
+    float2 v[D];
+    load_frame_row<D>(v, f, prb, ps, t, r, nz, n, P, ox, oy, cs, active);
+    frame_fwd<D>(v, T, slot, r);
+    if (active) {
+      float2 sh = shift[p];
+      float2* gp = g + p * D * D + r;
+#pragma unroll
+      for (int k = 0; k < D; ++k) gp[k * D] = cmul(v[k], sh);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ minf
+// sums over patches of the data term at ngam candidates; one partial per
+// (chunk, block) in `partial[(c*gridDim.x + b)*ngam + j]`.
+#define PT_MAX_GAM 8
+template <int D>
+__global__ __launch_bounds__(WAVE) void k_minf(double* __restrict__ partial,
+                                               const float2* __restrict__ f,
+                                               const float2* __restrict__ dir,
+                                               const float* __restrict__ data,
+                                               const float2* __restrict__ prb,
+                                               const int2* __restrict__ pos,
+                                               const double* __restrict__ gammas, int ngam,
+                                               int chunk, int nz, int n, int S, int P, int ox,
+                                               int oy, float cs, float cp, float cd, int model,
+                                               int lagr) {
+  constexpr int G = WAVE / D;
+  __shared__ float2 T[G * D * (D + 1)];
+  __shared__ double red[PT_MAX_GAM * 16];
+  const int lane = lane_id(), slot = lane / D, r = lane % D;
+  const int c = blockIdx.y;
+  float gam[PT_MAX_GAM];
+  for (int j = 0; j < PT_MAX_GAM; ++j) gam[j] = j < ngam ? (float)gammas[c * ngam + j] : 0.f;
+  double acc[PT_MAX_GAM];
+  for (int j = 0; j < PT_MAX_GAM; ++j) acc[j] = 0.0;
+  const long npatch = (long)chunk * S;
+  const long base = (long)c * chunk * S;
+  for (long grp = blockIdx.x; grp * G < npatch; grp += gridDim.x) {
+    const long pl = grp * G + slot;
+    const bool active = pl < npatch;
+    const long p = base + pl;
+    int2 ps = active ? pos[p] : make_int2(-1, -1);
+    const int t = active ? (int)(p / S) : 0;
+    float2 a[D], b[D];
+    load_frame_row<D>(a, f, prb, ps, t, r, nz, n, P, ox, oy, cs, active);
+    frame_fwd<D>(a, T, slot, r);
+    if (dir) {
+      load_frame_row<D>(b, dir, prb, ps, t, r, nz, n, P, ox, oy, cs, active);
+      frame_fwd<D>(b, T, slot, r);
+    } else {
+#pragma unroll
+      for (int k = 0; k < D; ++k) b[k] = make_float2(0.f, 0.f);
+    }
+    if (active) {
+      const float* dp = data + p * D * D + r;
+      float part[PT_MAX_GAM];
+      for (int j = 0; j < PT_MAX_GAM; ++j) part[j] = 0.f;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const float d = dp[k * D] * cd;
+        const float2 A = cscale(a[k], cp), Bv = cscale(b[k], cp);
+        const float aa = A.x * A.x + A.y * A.y;
+        const float ab = A.x * Bv.x + A.y * Bv.y;
+        const float bb = Bv.x * Bv.x + Bv.y * Bv.y;
+        float dterm = 0.f;
+        if (lagr && model == PT_MODEL_POISSON) dterm = d - 2.f * d * logf(fmaxf(sqrtf(d), 1e-32f));
+        for (int j = 0; j < ngam; ++j) {
+          const float gj = gam[j];
+          const float s = fmaxf(aa + gj * (2.f * ab + gj * bb), 0.f);
+          const float m = sqrtf(s);
+          float term;
+          if (model == PT_MODEL_POISSON) {
+            term = s - 2.f * d * logf(fmaxf(m, 1e-32f)) - dterm;
+          } else {
+            const float e = m - sqrtf(d);
+            term = e * e;
+          }
+          part[j] += term;
+        }
+      }
+      for (int j = 0; j < ngam; ++j) acc[j] += (double)part[j];
+    }
+  }
+  // block (single wave) reduction
+  for (int j = 0; j < ngam; ++j) {
+    double v = wave_sum(acc[j]);
+    if (lane == 0) partial[((long)c * gridDim.x + blockIdx.x) * ngam + j] = v;
+  }
+}
+
+__global__ void k_reduce_partials(double* __restrict__ out, const double* __restrict__ partial,
+                                  int nblk, int ngam) {
+  // one thread per (chunk, gamma): fixed-order sum over blocks
+  const int c = blockIdx.x;
+  const int j = threadIdx.x;
+  if (j >= ngam) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[((long)c * nblk + b) * ngam + j];
+  out[c * ngam + j] = s;
+}
+
+// ------------------------------------------------------------------ tiles
+enum { TILE_ADJ = 0, TILE_GRAD = 1 };
+
+template <int D, int MODE>
+__global__ __launch_bounds__(WAVE) void k_tile(
+    float2* __restrict__ scratch, const float2* __restrict__ gin, const float2* __restrict__ f,
+    const float* __restrict__ data, const float2* __restrict__ prb, const int2* __restrict__ pos,
+    const float2* __restrict__ shift, const int* __restrict__ tile_start,
+    const int* __restrict__ tile_patch, const int* __restrict__ tile_slot, int nty, int ntx,
+    int B, int EB, int nz, int n, int S, int P, int ox, int oy, float cs, float cp, float cd,
+    float out_scale, int model) {
+  constexpr int G = WAVE / D;
+  extern __shared__ float2 lds[];
+  float2* T = lds;                          // G*D*(D+1)
+  float2* acc = lds + G * D * (D + 1);      // EB*EB
+  const long tile = blockIdx.x;
+  const int slotid = tile_slot[tile];
+  if (slotid < 0) return;
+  const int lane = lane_id(), slot = lane / D, r = lane % D;
+  const int tpa = nty * ntx;
+  const int t = (int)(tile / tpa);
+  const int rem = (int)(tile - (long)t * tpa);
+  const int y0 = (rem / ntx) * B, x0 = (rem % ntx) * B;
+  for (int i = lane; i < EB * EB; i += WAVE) acc[i] = make_float2(0.f, 0.f);
+  __syncthreads();
+  const int beg = tile_start[tile], end = tile_start[tile + 1];
+  const int ix = r - ox;
+  for (int i0 = beg; i0 < end; i0 += G) {
+    const int ii = i0 + slot;
+    const bool active = ii < end;
+    const long p = active ? tile_patch[ii] : 0;
+    const int2 ps = active ? pos[p] : make_int2(-1, -1);
+    float2 v[D];
+    if constexpr (MODE == TILE_ADJ) {
+      // lane c loads far-field column c, removes the per-patch phase
+      if (active) {
+        const float2 sh = shift[p];
+        const float2* gp = gin + p * D * D + r;
+#pragma unroll
+        for (int k = 0; k < D; ++k) v[k] = cmulc(gp[k * D], sh);
+      } else {
+#pragma unroll
+        for (int k = 0; k < D; ++k) v[k] = make_float2(0.f, 0.f);
+      }
+    } else {
+      load_frame_row<D>(v, f, prb, ps, t, r, nz, n, P, ox, oy, cs, active);
+      frame_fwd<D>(v, T, slot, r);
+      // residual in the unshifted frame (the per-patch unimodular factor of
+      // kernels.cuh:319-355 cancels: r(s*a) = s*r(a), F* removes s again)
+      if (active) {
+        const float* dp = data + p * D * D + r;
+        const float2 sh = shift[p];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          const float d = dp[k * D] * cd;
+          const float2 a = cscale(v[k], cp);
+          const float aa = a.x * a.x + a.y * a.y;
+          float2 res;
+          if (model == PT_MODEL_POISSON) {
+            const float q = d / (aa + 1e-32f);
+            res = make_float2(a.x - q * a.x, a.y - q * a.y);
+          } else {
+            const float m = sqrtf(aa);
+            const float sd = sqrtf(d);
+            if (m > 0.f) {
+              const float q = sd / m;
+              res = make_float2(a.x - q * a.x, a.y - q * a.y);
+            } else {
+              // angle(0) = 0 in the shifted frame: r = -sqrt(d) there
+              res = make_float2(-sd * sh.x, sd * sh.y);
+            }
+          }
+          v[k] = res;
+        }
+      }
+    }
+    frame_inv<D>(v, T, slot, r);
+    // overlap-add of c*conj(prb)*frame into the LDS tile
+    if (active && ps.x >= 0 && ix >= 0 && ix < P) {
+      const float2* pp = prb + (long)ix * P - oy;
+      const int col = ps.x + ix - x0;
+      const int rowb = ps.y - oy - y0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const int iy = k - oy;
+        if (iy >= 0 && iy < P) {
+          const float2 val = cscale(cmulc(v[k], pp[k]), out_scale);
+          float2* dst = acc + (rowb + k) * EB + col;
+          atomicAdd(&dst->x, val.x);
+          atomicAdd(&dst->y, val.y);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  float2* out = scratch + (long)slotid * EB * EB;
+  for (int i = lane; i < EB * EB; i += WAVE) out[i] = acc[i];
+}
+
+template <int MODE>
+__global__ void k_fold(float2* __restrict__ out, const float2* __restrict__ scratch,
+                       const int* __restrict__ tile_slot, const float2* __restrict__ psi,
+                       const float2* __restrict__ hh, const float2* __restrict__ lamd, float rho,
+                       int pt, int nz, int n, int nty, int ntx, int B, int EB) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)pt * nz * n;
+  if (idx >= total) return;
+  const int x = (int)(idx % n);
+  const long tz = idx / n;
+  const int y = (int)(tz % nz);
+  const int t = (int)(tz / nz);
+  float2 s = make_float2(0.f, 0.f);
+  // tiles whose extended region [t*B, t*B + EB) contains the pixel, in
+  // ascending order (fixed summation order)
+  const int ylo = y - EB + 1, xlo = x - EB + 1;
+  const int ty0 = ylo <= 0 ? 0 : (ylo + B - 1) / B, ty1 = min(y / B, nty - 1);
+  const int tx0 = xlo <= 0 ? 0 : (xlo + B - 1) / B, tx1 = min(x / B, ntx - 1);
+  for (int ty = ty0; ty <= ty1; ++ty) {
+    const int ly = y - ty * B;
+    for (int tx = tx0; tx <= tx1; ++tx) {
+      const int lx = x - tx * B;
+      const int sl = tile_slot[((long)t * nty + ty) * ntx + tx];
+      if (sl < 0) continue;
+      s = cadd(s, scratch[(long)sl * EB * EB + ly * EB + lx]);
+    }
+  }
+  if constexpr (MODE == TILE_GRAD) {
+    // grad -= rho*(h - psi + lamd/rho)   (solver.py:225)
+    const float2 h = hh[idx], p = psi[idx], l = lamd[idx];
+    const float ir = 1.0f / rho;
+    const float2 q = make_float2(h.x - p.x + l.x * ir, h.y - p.y + l.y * ir);
+    s = make_float2(s.x - rho * q.x, s.y - rho * q.y);
+  }
+  out[idx] = s;
+}
+
+// ------------------------------------------------------------------ set-up kernels
+__global__ void k_setup_pos(int2* __restrict__ pos, float2* __restrict__ shift,
+                            const float* __restrict__ scan, long np) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const float sx = scan[i], sy = scan[np + i];
+  const float rx = roundf(sx), ry = roundf(sy);
+  pos[i] = make_int2((int)rx, (int)ry);
+  // takeshifts (kernels.cuh:306-317): e^{2 pi i frac(x)} * e^{2 pi i frac(y)}
+  double ax = 2.0 * kPi * (double)(sx - rx), ay = 2.0 * kPi * (double)(sy - ry);
+  double cx = cos(ax), sxn = sin(ax), cy = cos(ay), syn = sin(ay);
+  float2 a = make_float2((float)cx, (float)sxn), b = make_float2((float)cy, (float)syn);
+  shift[i] = cmul(a, b);
+}
+
+int launch_blocks(long work, int per_block, int cap) {
+  long b = (work + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+
+template <int D>
+int run_fwd(pt_ptycho* h, float2* g, const float2* f, float scale, hipStream_t st) {
+  constexpr int G = WAVE / D;
+  long np = (long)h->pt * h->S;
+  int nb = launch_blocks(np, G, 1 << 20);
+  hipLaunchKernelGGL(k_fwd<D>, dim3(nb), dim3(WAVE), 0, st, g, f, h->prb, h->pos, h->shift, h->pt,
+                     h->nz, h->n, h->S, h->P, h->ox, h->oy, h->c * scale);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+template <int D, int MODE>
+int run_tiles(pt_ptycho* h, const float2* gin, const float2* f, const float* data, float cp,
+              float cd, float out_scale, int model, hipStream_t st) {
+  constexpr int G = WAVE / D;
+  size_t lds = sizeof(float2) * ((size_t)G * D * (D + 1) + (size_t)h->EB * h->EB);
+  hipLaunchKernelGGL((k_tile<D, MODE>), dim3((unsigned)h->ntiles), dim3(WAVE), lds, st, h->scratch,
+                     gin, f, data, h->prb, h->pos, h->shift, h->tile_start, h->tile_patch,
+                     h->tile_slot, h->nty, h->ntx, h->B, h->EB, h->nz, h->n, h->S, h->P, h->ox,
+                     h->oy, h->c, cp, cd, out_scale, model);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+template <int D>
+int run_minf(pt_ptycho* h, double* sums, const float2* f, const float2* dir, const float* data,
+             const double* gammas, int ngam, int chunk, float cp, float cd, int model, int lagr,
+             hipStream_t st) {
+  constexpr int G = WAVE / D;
+  const int nchunk = h->pt / chunk;
+  long np = (long)chunk * h->S;
+  int nb = launch_blocks(np, G, std::max(1, 4096 / nchunk));
+  size_t need = (size_t)nchunk * nb * ngam;
+  if (need > h->partial_cap) {
+    if (h->partial) PT_HIP(hipFree(h->partial));
+    h->partial = nullptr;
+    PT_HIP(hipMalloc(&h->partial, need * sizeof(double)));
+    h->partial_cap = need;
+  }
+  hipLaunchKernelGGL(k_minf<D>, dim3(nb, nchunk), dim3(WAVE), 0, st, h->partial, f, dir, data,
+                     h->prb, h->pos, gammas, ngam, chunk, h->nz, h->n, h->S, h->P, h->ox, h->oy,
+                     h->c, cp, cd, model, lagr);
+  PT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_partials, dim3(nchunk), dim3(64), 0, st, sums, h->partial, nb, ngam);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+template <int MODE>
+int run_fold(pt_ptycho* h, float2* out, const float2* psi, const float2* hh, const float2* lamd,
+             float rho, hipStream_t st) {
+  long total = (long)h->pt * h->nz * h->n;
+  int nb = (int)((total + 255) / 256);
+  hipLaunchKernelGGL(k_fold<MODE>, dim3(nb), dim3(256), 0, st, out, h->scratch, h->tile_slot, psi,
+                     hh, lamd, rho, h->pt, h->nz, h->n, h->nty, h->ntx, h->B, h->EB);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+#define PT_DISPATCH_D(D, CALL)                         \
+  switch (D) {                                         \
+    case 8: { constexpr int DD = 8; return CALL; }     \
+    case 16: { constexpr int DD = 16; return CALL; }   \
+    case 32: { constexpr int DD = 32; return CALL; }   \
+    case 64: { constexpr int DD = 64; return CALL; }   \
+    default: set_error("unsupported detector size"); return PT_ERR_SIZE; \
+  }
+
+int fwd_impl(pt_ptycho* h, void* g, const void* f, float scale, pt_stream stream) {
+  PT_REQUIRE(h && g && f, PT_ERR_ARG, "pt_ptycho_fwd: null argument");
+  PT_REQUIRE(h->ready, PT_ERR_STATE, "pt_ptycho_fwd: set_scan_probe not called");
+  hipStream_t st = (hipStream_t)stream;
+  PT_DISPATCH_D(h->D, (run_fwd<DD>(h, (float2*)g, (const float2*)f, scale, st)));
+}
+
+template <int MODE>
+int tiles_impl(pt_ptycho* h, float2* out, const float2* gin, const float2* f, const float* data,
+               const float2* hh, const float2* lamd, double rho, float cp, float cd,
+               float out_scale, int model, hipStream_t st) {
+  int rc;
+  switch (h->D) {
+    case 8: rc = run_tiles<8, MODE>(h, gin, f, data, cp, cd, out_scale, model, st); break;
+    case 16: rc = run_tiles<16, MODE>(h, gin, f, data, cp, cd, out_scale, model, st); break;
+    case 32: rc = run_tiles<32, MODE>(h, gin, f, data, cp, cd, out_scale, model, st); break;
+    case 64: rc = run_tiles<64, MODE>(h, gin, f, data, cp, cd, out_scale, model, st); break;
+    default: set_error("unsupported detector size"); return PT_ERR_SIZE;
+  }
+  if (rc) return rc;
+  return run_fold<MODE>(h, out, f, hh, lamd, (float)rho, st);
+}
+
+// LDS budget per single-wave workgroup used to size the overlap-add tiles
+constexpr size_t kTileLdsBudget = 40 * 1024;
+
+}  // namespace
+}  // namespace pt
+
+using namespace pt;
+
+extern "C" int pt_ptycho_create(int ptheta, int nz, int n, int ntheta0, int nscan, int detx,
+                                int dety, int nprb, pt_ptycho** out) {
+  (void)ntheta0;
+  PT_REQUIRE(out, PT_ERR_ARG, "pt_ptycho_create: null output handle");
+  *out = nullptr;
+  PT_REQUIRE(ptheta > 0 && nz > 0 && n > 0 && nscan > 0 && nprb > 0, PT_ERR_SIZE,
+             "pt_ptycho_create: sizes must be positive");
+  PT_REQUIRE(detx == dety, PT_ERR_SIZE, "pt_ptycho_create: detx must equal dety");
+  PT_REQUIRE(detx == 8 || detx == 16 || detx == 32 || detx == 64, PT_ERR_SIZE,
+             "pt_ptycho_create: detector size must be 8, 16, 32 or 64");
+  PT_REQUIRE(nprb <= detx && nprb <= n && nprb <= nz, PT_ERR_SIZE,
+             "pt_ptycho_create: probe larger than detector or object");
+  pt_ptycho* h = new pt_ptycho();
+  h->pt = ptheta; h->nz = nz; h->n = n; h->S = nscan; h->D = detx; h->P = nprb;
+  h->ox = (detx - nprb) / 2;
+  h->oy = (dety - nprb) / 2;
+  h->c = 1.0f / sqrtf((float)(detx * dety));
+  // tile size from the LDS budget
+  const int G = 64 / detx;
+  size_t tbytes = sizeof(float2) * (size_t)G * detx * (detx + 1);
+  long maxe = (long)std::sqrt((double)(kTileLdsBudget - std::min(tbytes, kTileLdsBudget - 4096)) / sizeof(float2));
+  int EB = (int)std::max<long>(maxe, nprb + 7);
+  int B = EB - (nprb - 1);
+  h->B = B; h->EB = EB;
+  h->ntx = (n - nprb) / B + 1;
+  h->nty = (nz - nprb) / B + 1;
+  h->ntiles = (long)ptheta * h->ntx * h->nty;
+  long np = (long)ptheta * nscan;
+  auto fail = [&](hipError_t e, const char* what) {
+    set_error(std::string("pt_ptycho_create: ") + what + ": " + hipGetErrorString(e));
+    pt_ptycho_destroy(h);
+    return PT_ERR_HIP;
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&h->prb, sizeof(float2) * nprb * nprb))) return fail(e, "prb");
+  if ((e = hipMalloc(&h->pos, sizeof(int2) * np))) return fail(e, "pos");
+  if ((e = hipMalloc(&h->shift, sizeof(float2) * np))) return fail(e, "shift");
+  if ((e = hipMalloc(&h->tile_start, sizeof(int) * (h->ntiles + 1)))) return fail(e, "tiles");
+  if ((e = hipMalloc(&h->tile_patch, sizeof(int) * std::max<long>(np, 1)))) return fail(e, "tiles");
+  if ((e = hipMalloc(&h->tile_slot, sizeof(int) * h->ntiles))) return fail(e, "tiles");
+  *out = h;
+  return PT_OK;
+}
+
+extern "C" int pt_ptycho_destroy(pt_ptycho* h) {
+  if (!h) return PT_OK;
+  (void)hipFree(h->prb);
+  (void)hipFree(h->pos);
+  (void)hipFree(h->shift);
+  (void)hipFree(h->tile_start);
+  (void)hipFree(h->tile_patch);
+  (void)hipFree(h->tile_slot);
+  (void)hipFree(h->scratch);
+  (void)hipFree(h->partial);
+  delete h;
+  return PT_OK;
+}
+
+extern "C" int pt_ptycho_set_scan_probe(pt_ptycho* h, const float* scan, const void* prb,
+                                        pt_stream stream) {
+  PT_REQUIRE(h && scan && prb, PT_ERR_ARG, "pt_ptycho_set_scan_probe: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const long np = (long)h->pt * h->S;
+  PT_HIP(hipMemcpyAsync(h->prb, prb, sizeof(float2) * h->P * h->P, hipMemcpyDeviceToDevice, st));
+  int nb = (int)((np + 255) / 256);
+  hipLaunchKernelGGL(k_setup_pos, dim3(nb), dim3(256), 0, st, h->pos, h->shift, scan, np);
+  PT_LAUNCH_CHECK();
+  // bin valid patches by tile (host; geometry is static between calls)
+  std::vector<int2> pos(np);
+  PT_HIP(hipMemcpyAsync(pos.data(), h->pos, sizeof(int2) * np, hipMemcpyDeviceToHost, st));
+  PT_HIP(hipStreamSynchronize(st));
+  std::vector<int> cnt(h->ntiles + 1, 0);
+  for (long i = 0; i < np; ++i) {
+    const int2 p = pos[i];
+    if (p.x < 0 || p.y < 0) continue;
+    if (p.x > h->n - h->P || p.y > h->nz - h->P) {
+      set_error("pt_ptycho_set_scan_probe: scan position places the probe outside the object");
+      h->ready = false;
+      return PT_ERR_ARG;
+    }
+    const long t = i / h->S;
+    const long tile = (t * h->nty + p.y / h->B) * h->ntx + p.x / h->B;
+    cnt[tile + 1]++;
+  }
+  for (long i = 0; i < h->ntiles; ++i) cnt[i + 1] += cnt[i];
+  std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+  std::vector<int> list(std::max<long>(cnt[h->ntiles], 1));
+  for (long i = 0; i < np; ++i) {
+    const int2 p = pos[i];
+    if (p.x < 0 || p.y < 0) continue;
+    const long t = i / h->S;
+    const long tile = (t * h->nty + p.y / h->B) * h->ntx + p.x / h->B;
+    list[fillp[tile]++] = (int)i;
+  }
+  std::vector<int> slot(h->ntiles, -1);
+  long ns = 0;
+  for (long i = 0; i < h->ntiles; ++i)
+    if (cnt[i + 1] > cnt[i]) slot[i] = (int)ns++;
+  h->nslots = ns;
+  size_t need = (size_t)std::max<long>(ns, 1) * h->EB * h->EB;
+  if (need > h->scratch_cap) {
+    if (h->scratch) PT_HIP(hipFree(h->scratch));
+    h->scratch = nullptr;
+    PT_HIP(hipMalloc(&h->scratch, need * sizeof(float2)));
+    h->scratch_cap = need;
+  }
+  PT_HIP(hipMemcpyAsync(h->tile_start, cnt.data(), sizeof(int) * (h->ntiles + 1),
+                        hipMemcpyHostToDevice, st));
+  PT_HIP(hipMemcpyAsync(h->tile_patch, list.data(), sizeof(int) * list.size(),
+                        hipMemcpyHostToDevice, st));
+  PT_HIP(hipMemcpyAsync(h->tile_slot, slot.data(), sizeof(int) * h->ntiles,
+                        hipMemcpyHostToDevice, st));
+  PT_HIP(hipStreamSynchronize(st));
+  h->ready = true;
+  return PT_OK;
+}
+
+extern "C" int pt_ptycho_fwd(pt_ptycho* h, void* g, const void* f, pt_stream stream) {
+  return fwd_impl(h, g, f, 1.0f, stream);
+}
+extern "C" int pt_ptycho_fwd_scaled(pt_ptycho* h, void* g, const void* f, float scale,
+                                    pt_stream stream) {
+  return fwd_impl(h, g, f, scale, stream);
+}
+
+static int adj_impl(pt_ptycho* h, void* f, const void* g, float scale, pt_stream stream) {
+  PT_REQUIRE(h && g && f, PT_ERR_ARG, "pt_ptycho_adj: null argument");
+  PT_REQUIRE(h->ready, PT_ERR_STATE, "pt_ptycho_adj: set_scan_probe not called");
+  return tiles_impl<TILE_ADJ>(h, (float2*)f, (const float2*)g, nullptr, nullptr, nullptr,
+                              nullptr, 0.0, 1.0f, 1.0f, h->c * scale, 0, (hipStream_t)stream);
+}
+extern "C" int pt_ptycho_adj(pt_ptycho* h, void* f, const void* g, pt_stream stream) {
+  return adj_impl(h, f, g, 1.0f, stream);
+}
+extern "C" int pt_ptycho_adj_scaled(pt_ptycho* h, void* f, const void* g, float scale,
+                                    pt_stream stream) {
+  return adj_impl(h, f, g, scale, stream);
+}
+
+extern "C" int pt_ptycho_grad(pt_ptycho* h, void* grad, const void* psi, const float* data,
+                              const void* hh, const void* lamd, double rho, float coefptycho,
+                              float coefdata, int model, pt_stream stream) {
+  PT_REQUIRE(h && grad && psi && data && hh && lamd, PT_ERR_ARG, "pt_ptycho_grad: null argument");
+  PT_REQUIRE(h->ready, PT_ERR_STATE, "pt_ptycho_grad: set_scan_probe not called");
+  PT_REQUIRE(model == PT_MODEL_POISSON || model == PT_MODEL_GAUSSIAN, PT_ERR_ARG,
+             "pt_ptycho_grad: bad model");
+  PT_REQUIRE(rho > 0, PT_ERR_ARG, "pt_ptycho_grad: rho must be positive");
+  return tiles_impl<TILE_GRAD>(h, (float2*)grad, nullptr, (const float2*)psi, data,
+                               (const float2*)hh, (const float2*)lamd, rho, coefptycho, coefdata,
+                               h->c * coefptycho, model, (hipStream_t)stream);
+}
+
+extern "C" int pt_ptycho_minf(pt_ptycho* h, double* sums, const void* psi, const void* dir,
+                              const float* data, const double* gammas, int ngam, int chunk,
+                              float coefptycho, float coefdata, int model, int lagr,
+                              pt_stream stream) {
+  PT_REQUIRE(h && sums && psi && data && gammas, PT_ERR_ARG, "pt_ptycho_minf: null argument");
+  PT_REQUIRE(h->ready, PT_ERR_STATE, "pt_ptycho_minf: set_scan_probe not called");
+  PT_REQUIRE(ngam >= 1 && ngam <= PT_MAX_GAM, PT_ERR_ARG, "pt_ptycho_minf: 1 <= ngam <= 8");
+  PT_REQUIRE(chunk > 0 && h->pt % chunk == 0, PT_ERR_SIZE,
+             "pt_ptycho_minf: chunk must divide the number of angles");
+  PT_REQUIRE(model == PT_MODEL_POISSON || model == PT_MODEL_GAUSSIAN, PT_ERR_ARG,
+             "pt_ptycho_minf: bad model");
+  hipStream_t st = (hipStream_t)stream;
+  PT_DISPATCH_D(h->D, (run_minf<DD>(h, sums, (const float2*)psi, (const float2*)dir, data, gammas,
+                                     ngam, chunk, coefptycho, coefdata, model, lagr, st)));
+}

@@ -1,0 +1,709 @@
+// Solver kernels for gfx950: the element-wise and reduction chains of
+// src/ptychotomo/solver.py (cg_tomo, cg_ptycho, solve_reg, takexi,
+// update_penalty, take_lagr), each as one fused pass.  Reductions accumulate
+// in FP64 with a fixed grid and a fixed-order second pass (deterministic).
+#include <cmath>
+#include <mutex>
+#include <string>
+
+#include "common.hpp"
+
+namespace pt {
+
+static thread_local std::string g_last_error = "no error";
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+
+constexpr int RB = 256;          // threads per reduction block
+constexpr int RMAXBLK = 512;     // blocks per segment cap
+
+// scratch for partial sums (grown on demand, one per process)
+struct Partials {
+  double* ptr = nullptr;
+  size_t cap = 0;
+  std::mutex mu;
+} g_part;
+
+int partial_buf(size_t n, double** out) {
+  std::lock_guard<std::mutex> lk(g_part.mu);
+  if (n > g_part.cap) {
+    if (g_part.ptr) PT_HIP(hipFree(g_part.ptr));
+    g_part.ptr = nullptr;
+    PT_HIP(hipMalloc(&g_part.ptr, n * sizeof(double)));
+    g_part.cap = n;
+  }
+  *out = g_part.ptr;
+  return PT_OK;
+}
+
+template <int NV, typename F>
+__global__ __launch_bounds__(RB) void k_seg_reduce(double* __restrict__ partial, long seglen, F f) {
+  __shared__ double sm[NV * 16];
+  const int seg = blockIdx.y;
+  const long base = (long)seg * seglen;
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < seglen;
+       i += (long)gridDim.x * blockDim.x)
+    f(base + i, v);
+  block_sum<NV>(v, sm);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partial[((long)seg * gridDim.x + blockIdx.x) * NV + k] = v[k];
+  }
+}
+
+__global__ void k_finish(double* __restrict__ out, const double* __restrict__ partial, int nblk,
+                         int nv) {
+  const int seg = blockIdx.x, k = threadIdx.x;
+  if (k >= nv) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[((long)seg * nblk + b) * nv + k];
+  out[seg * nv + k] = s;
+}
+
+template <int NV, typename F>
+int seg_reduce(double* out, long len, int nseg, F f, hipStream_t st) {
+  PT_REQUIRE(nseg > 0 && len % nseg == 0, PT_ERR_SIZE, "reduction: nseg must divide len");
+  const long seglen = len / nseg;
+  int nblk = (int)std::min<long>(RMAXBLK, std::max<long>(1, (seglen + RB * 4 - 1) / (RB * 4)));
+  nblk = std::max(1, std::min(nblk, std::max(1, 2048 / nseg)));
+  double* part;
+  int rc = partial_buf((size_t)nseg * nblk * NV, &part);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_seg_reduce<NV, F>), dim3(nblk, nseg), dim3(RB), 0, st, part, seglen, f);
+  PT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_finish, dim3(nseg), dim3(64), 0, st, out, part, nblk, NV);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+template <typename F>
+__global__ void k_map(long len, F f) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < len;
+       i += (long)gridDim.x * blockDim.x)
+    f(i);
+}
+
+template <typename F>
+int map(long len, F f, hipStream_t st) {
+  long nb = (len + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL((k_map<F>), dim3((unsigned)nb), dim3(256), 0, st, len, f);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+// ------------------------------------------------------------------ functors
+struct FDot {
+  const float2* x; const float2* y;
+  __device__ void operator()(long i, double (&v)[2]) const {
+    const float2 a = x[i], b = y[i];
+    v[0] += (double)a.x * b.x + (double)a.y * b.y;
+    v[1] += (double)a.x * b.y - (double)a.y * b.x;
+  }
+};
+struct FNorm2 {
+  const float2* x;
+  __device__ void operator()(long i, double (&v)[1]) const {
+    const float2 a = x[i];
+    v[0] += (double)a.x * a.x + (double)a.y * a.y;
+  }
+};
+struct FNorm2Lin {
+  const float2* x; const float2* y; float ar, ai, br, bi;
+  __device__ void operator()(long i, double (&v)[1]) const {
+    const float2 a = x[i];
+    const float2 b = y ? y[i] : make_float2(0.f, 0.f);
+    const float re = ar * a.x - ai * a.y + br * b.x - bi * b.y;
+    const float im = ar * a.y + ai * a.x + br * b.y + bi * b.x;
+    v[0] += (double)re * re + (double)im * im;
+  }
+};
+struct FDyReduce {
+  const float2* g; const float2* g0; const float2* d;
+  __device__ void operator()(long i, double (&v)[3]) const {
+    const float2 a = g[i], b = g0[i], c = d[i];
+    const float dx = a.x - b.x, dy = a.y - b.y;
+    v[0] += (double)a.x * a.x + (double)a.y * a.y;
+    v[1] += (double)c.x * dx + (double)c.y * dy;
+    v[2] += (double)c.x * dy - (double)c.y * dx;
+  }
+};
+
+struct FLincomb {
+  float2* z; const double* a; const float2* x; const double* b; const float2* y; long seglen;
+  __device__ void operator()(long i) const {
+    const long s = i / seglen;
+    const float2 av = make_float2((float)a[2 * s], (float)a[2 * s + 1]);
+    float2 r = cmul(av, x[i]);
+    if (y) {
+      const float2 bv = make_float2((float)b[2 * s], (float)b[2 * s + 1]);
+      r = cadd(r, cmul(bv, y[i]));
+    }
+    z[i] = r;
+  }
+};
+struct FDyUpdate {
+  float2* d; const float2* g; const double* beta; int first; long seglen;
+  __device__ void operator()(long i) const {
+    const float2 gv = g[i];
+    if (first) {
+      d[i] = make_float2(-gv.x, -gv.y);
+    } else {
+      const long s = i / seglen;
+      const float2 bv = make_float2((float)beta[2 * s], (float)beta[2 * s + 1]);
+      const float2 t = cmul(bv, d[i]);
+      d[i] = make_float2(t.x - gv.x, t.y - gv.y);
+    }
+  }
+};
+
+// fwd_reg: res[0] x-diff, res[1] y-diff, res[2] z-diff, times 2/sqrt(3)
+struct FRegFwd {
+  float2* out; const float2* u; int nz, ny, nx;
+  __device__ void operator()(long i) const {
+    const long plane = (long)nz * ny * nx;
+    const int x = (int)(i % nx);
+    const long r = i / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    const float c = 1.15470053837925152902f;  // 2/sqrt(3)
+    const float2 v = u[i];
+    float2 a = make_float2(0.f, 0.f), b = a, e = a;
+    if (x < nx - 1) { float2 w = u[i + 1]; a = make_float2((w.x - v.x) * c, (w.y - v.y) * c); }
+    if (y < ny - 1) { float2 w = u[i + nx]; b = make_float2((w.x - v.x) * c, (w.y - v.y) * c); }
+    if (z < nz - 1) { float2 w = u[i + (long)ny * nx]; e = make_float2((w.x - v.x) * c, (w.y - v.y) * c); }
+    out[i] = a;
+    out[plane + i] = b;
+    out[2 * plane + i] = e;
+  }
+};
+// adj_reg (solver.py:137-146)
+struct FRegAdj {
+  float2* out; const float2* g; int nz, ny, nx;
+  __device__ void operator()(long i) const {
+    const long plane = (long)nz * ny * nx;
+    const int x = (int)(i % nx);
+    const long r = i / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    const float c = -1.15470053837925152902f;
+    const float2* g0 = g;
+    const float2* g1 = g + plane;
+    const float2* g2 = g + 2 * plane;
+    float2 s = g0[i];
+    if (x > 0) s = csub(s, g0[i - 1]);
+    float2 t = g1[i];
+    if (y > 0) t = csub(t, g1[i - nx]);
+    s = cadd(s, t);
+    float2 w = g2[i];
+    if (z > 0) w = csub(w, g2[i - (long)ny * nx]);
+    s = cadd(s, w);
+    out[i] = make_float2(s.x * c, s.y * c);
+  }
+};
+// solve_reg (solver.py:257-264)
+struct FRegSolve {
+  float2* out; const float2* u; const float2* mu; float itau, thr; int nz, ny, nx;
+  __device__ void operator()(long i) const {
+    const long plane = (long)nz * ny * nx;
+    const int x = (int)(i % nx);
+    const long r = i / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    const float c = 1.15470053837925152902f;
+    const float2 v = u[i];
+    float2 g[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+    if (x < nx - 1) { float2 w = u[i + 1]; g[0] = make_float2((w.x - v.x) * c, (w.y - v.y) * c); }
+    if (y < ny - 1) { float2 w = u[i + nx]; g[1] = make_float2((w.x - v.x) * c, (w.y - v.y) * c); }
+    if (z < nz - 1) { float2 w = u[i + (long)ny * nx]; g[2] = make_float2((w.x - v.x) * c, (w.y - v.y) * c); }
+    float ss = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float2 m = mu[k * plane + i];
+      g[k] = make_float2(g[k].x + m.x * itau, g[k].y + m.y * itau);
+      ss += g[k].x * g[k].x + g[k].y * g[k].y;
+    }
+    const float za = sqrtf(ss);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float2 o;
+      if (za <= thr) o = make_float2(0.f, 0.f);
+      else {
+        const float q = thr / za;
+        o = make_float2(g[k].x - q * g[k].x, g[k].y - q * g[k].y);
+      }
+      out[k * plane + i] = o;
+    }
+  }
+};
+
+struct FExpTomo {
+  float2* out; const float2* psi; float c; float2 ph;
+  __device__ void operator()(long i) const {
+    // exp(i*c*psi) = exp(-c*Im psi) * (cos(c*Re psi) + i sin(c*Re psi))
+    const float2 p = psi[i];
+    const float a = expf(-c * p.y);
+    float sn, cs;
+    sincosf(c * p.x, &sn, &cs);
+    out[i] = cmul(make_float2(a * cs, a * sn), ph);
+  }
+};
+
+__device__ __forceinline__ float2 mlog_c(float2 x) {
+  const float m = hypotf(x.x, x.y);
+  if (m < 1e-32f) return make_float2(logf(1e-32f), 0.f);
+  return make_float2(logf(m), atan2f(x.y, x.x));
+}
+
+struct FMlog {
+  float2* out; const float2* x;
+  __device__ void operator()(long i) const { out[i] = mlog_c(x[i]); }
+};
+
+struct FTakexiStats {
+  const float2* psi; const float2* lamd; float irho; int nz, n, r;
+  __device__ void operator()(long i, double (&v)[3]) const {
+    const int x = (int)(i % n);
+    const float2 p = psi[i];
+    if (x >= r && x < 2 * r) v[0] += (double)atan2f(p.y, p.x);
+    if (x >= n - 2 * r && x < n - r) v[1] += (double)atan2f(p.y, p.x);
+    const float2 l = lamd[i];
+    const float tx = p.x - l.x * irho, ty = p.y - l.y * irho;
+    const double m = (double)hypotf(tx, ty);
+    if (m > v[2]) v[2] = m;
+  }
+};
+
+struct FTakexi {
+  float2* xi0; float2* K; const float2* psi; const float2* lamd; float rho; float2 eph;
+  float kfac; float xfac;
+  __device__ void operator()(long i) const {
+    const float2 p = psi[i], l = lamd[i];
+    float2 t = make_float2(p.x - l.x / rho, p.y - l.y / rho);
+    t = cmul(t, eph);
+    const float2 lt = mlog_c(t);
+    // K = i*kfac*t
+    const float2 k = make_float2(-kfac * t.y, kfac * t.x);
+    K[i] = k;
+    // xi0 = K * (-i * logt) * xfac
+    const float2 m = make_float2(lt.y * xfac, -lt.x * xfac);
+    xi0[i] = cmul(k, m);
+  }
+};
+
+
+// cg_tomo residual: r = conj(K)*(K*Ru - xi0); v[0] += |K*Ru - xi0|^2
+struct FTomoResid {
+  float2* r; const float2* K; const float2* Ru; const float2* xi0;
+  __device__ void operator()(long i, double (&v)[1]) const {
+    const float2 k = K[i];
+    const float2 e = csub(cmul(k, Ru[i]), xi0[i]);
+    r[i] = cmul(cconj(k), e);
+    v[0] += (double)e.x * e.x + (double)e.y * e.y;
+  }
+};
+// cg_tomo line-search terms: w = K*Rd; v[0] += Re conj(K*Ru - xi0)*w, v[1] += |w|^2
+struct FTomoLs {
+  const float2* K; const float2* Ru; const float2* Rd; const float2* xi0;
+  __device__ void operator()(long i, double (&v)[2]) const {
+    const float2 k = K[i];
+    const float2 e = csub(cmul(k, Ru[i]), xi0[i]);
+    const float2 w = cmul(k, Rd[i]);
+    v[0] += (double)e.x * w.x + (double)e.y * w.y;
+    v[1] += (double)w.x * w.x + (double)w.y * w.y;
+  }
+};
+
+__device__ __forceinline__ void grad3(const float2* __restrict__ u, long i, int x, int y, int z,
+                                      int nz, int ny, int nx, float2 (&g)[3]) {
+  const float c = 1.15470053837925152902f;
+  const float2 v = u[i];
+  g[0] = g[1] = g[2] = make_float2(0.f, 0.f);
+  if (x < nx - 1) { float2 w = u[i + 1]; g[0] = make_float2((w.x - v.x) * c, (w.y - v.y) * c); }
+  if (y < ny - 1) { float2 w = u[i + nx]; g[1] = make_float2((w.x - v.x) * c, (w.y - v.y) * c); }
+  if (z < nz - 1) { float2 w = u[i + (long)ny * nx]; g[2] = make_float2((w.x - v.x) * c, (w.y - v.y) * c); }
+}
+
+// grad = rho_part + tau * J*(J u - xi1);  v[0] += |J u - xi1|^2
+struct FRegGrad {
+  float2* grad; const float2* u; const float2* xi1; float tau; int nz, ny, nx;
+  __device__ void operator()(long i, double (&v)[1]) const {
+    const long plane = (long)nz * ny * nx;
+    const int x = (int)(i % nx);
+    const long r = i / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    float2 g[3];
+    grad3(u, i, x, y, z, nz, ny, nx, g);
+    float2 w[3];
+    for (int k = 0; k < 3; ++k) {
+      w[k] = csub(g[k], xi1[k * plane + i]);
+      v[0] += (double)w[k].x * w[k].x + (double)w[k].y * w[k].y;
+    }
+    // adj_reg of w (solver.py:137-146): sum_k w_k(i) - w_k(i - e_k) (if i_k > 0)
+    float2 s = make_float2(w[0].x + w[1].x + w[2].x, w[0].y + w[1].y + w[2].y);
+    float2 gn[3];
+    if (x > 0) {
+      grad3(u, i - 1, x - 1, y, z, nz, ny, nx, gn);
+      s = csub(s, csub(gn[0], xi1[i - 1]));
+    }
+    if (y > 0) {
+      grad3(u, i - nx, x, y - 1, z, nz, ny, nx, gn);
+      s = csub(s, csub(gn[1], xi1[plane + i - nx]));
+    }
+    if (z > 0) {
+      const long off = (long)ny * nx;
+      grad3(u, i - off, x, y, z - 1, nz, ny, nx, gn);
+      s = csub(s, csub(gn[2], xi1[2 * plane + i - off]));
+    }
+    const float c = -1.15470053837925152902f * tau;
+    const float2 gv = grad[i];
+    grad[i] = make_float2(gv.x + c * s.x, gv.y + c * s.y);
+  }
+};
+// regularisation line-search terms: v[0] += Re conj(J u - xi1)*J d, v[1] += |J d|^2
+struct FRegLs {
+  const float2* u; const float2* d; const float2* xi1; int nz, ny, nx;
+  __device__ void operator()(long i, double (&v)[2]) const {
+    const long plane = (long)nz * ny * nx;
+    const int x = (int)(i % nx);
+    const long r = i / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    float2 gu[3], gd[3];
+    grad3(u, i, x, y, z, nz, ny, nx, gu);
+    grad3(d, i, x, y, z, nz, ny, nx, gd);
+    for (int k = 0; k < 3; ++k) {
+      const float2 e = csub(gu[k], xi1[k * plane + i]);
+      v[0] += (double)e.x * gd[k].x + (double)e.y * gd[k].y;
+      v[1] += (double)gd[k].x * gd[k].x + (double)gd[k].y * gd[k].y;
+    }
+  }
+};
+// cg_ptycho rho terms per chunk with r0 = h - psi + lamd/rho:
+// v[0] += |r0|^2, v[1] += Re conj(r0)*d, v[2] += |d|^2
+struct FRhoTerms {
+  const float2* h; const float2* psi; const float2* lamd; const float2* d; float rho;
+  __device__ void operator()(long i, double (&v)[3]) const {
+    const float2 a = h[i], p = psi[i], l = lamd[i], dd = d[i];
+    const float2 r0 = make_float2(a.x - p.x + l.x / rho, a.y - p.y + l.y / rho);
+    v[0] += (double)r0.x * r0.x + (double)r0.y * r0.y;
+    v[1] += (double)r0.x * dd.x + (double)r0.y * dd.y;
+    v[2] += (double)dd.x * dd.x + (double)dd.y * dd.y;
+  }
+};
+// sum conj(a)*(b - c)
+struct FDotDiff {
+  const float2* a; const float2* b; const float2* c;
+  __device__ void operator()(long i, double (&v)[2]) const {
+    const float2 x = a[i], y = csub(b[i], c[i]);
+    v[0] += (double)x.x * y.x + (double)x.y * y.y;
+    v[1] += (double)x.x * y.y - (double)x.y * y.x;
+  }
+};
+// sum_i sqrt(sum_k |phi_k(i)|^2)
+struct FTvNorm {
+  const float2* phi; long plane;
+  __device__ void operator()(long i, double (&v)[1]) const {
+    float s = 0.f;
+    for (int k = 0; k < 3; ++k) {
+      const float2 a = phi[k * plane + i];
+      s += a.x * a.x + a.y * a.y;
+    }
+    v[0] += (double)sqrtf(s);
+  }
+};
+struct FLincomb3 {
+  float2* z; float2 a; const float2* x; float2 b; const float2* y; float2 c; const float2* w;
+  __device__ void operator()(long i) const {
+    float2 r = cmul(a, x[i]);
+    if (y) r = cadd(r, cmul(b, y[i]));
+    if (w) r = cadd(r, cmul(c, w[i]));
+    z[i] = r;
+  }
+};
+struct FAxpySeg {
+  float2* y; const double* a; const float2* x; long seglen;
+  __device__ void operator()(long i) const {
+    const long s = i / seglen;
+    const float2 av = make_float2((float)a[2 * s], (float)a[2 * s + 1]);
+    y[i] = cadd(y[i], cmul(av, x[i]));
+  }
+};
+}  // namespace
+}  // namespace pt
+
+using namespace pt;
+
+extern "C" const char* pt_last_error(void) { return g_last_error.c_str(); }
+extern "C" int pt_abi_version(void) { return 1; }
+
+extern "C" int pt_k_dot(double* out, const void* x, const void* y, long len, int nseg,
+                        pt_stream stream) {
+  PT_REQUIRE(out && x && y, PT_ERR_ARG, "pt_k_dot: null argument");
+  return seg_reduce<2>(out, len, nseg, FDot{(const float2*)x, (const float2*)y}, (hipStream_t)stream);
+}
+extern "C" int pt_k_norm2(double* out, const void* x, long len, int nseg, pt_stream stream) {
+  PT_REQUIRE(out && x, PT_ERR_ARG, "pt_k_norm2: null argument");
+  return seg_reduce<1>(out, len, nseg, FNorm2{(const float2*)x}, (hipStream_t)stream);
+}
+extern "C" int pt_k_norm2_lin(double* out, const void* x, const void* y, double ar, double ai,
+                              double br, double bi, long len, int nseg, pt_stream stream) {
+  PT_REQUIRE(out && x, PT_ERR_ARG, "pt_k_norm2_lin: null argument");
+  return seg_reduce<1>(out, len, nseg,
+                       FNorm2Lin{(const float2*)x, (const float2*)y, (float)ar, (float)ai,
+                                 (float)br, (float)bi},
+                       (hipStream_t)stream);
+}
+extern "C" int pt_k_lincomb(void* z, const double* a_dev, const void* x, const double* b_dev,
+                            const void* y, long len, int nseg, pt_stream stream) {
+  PT_REQUIRE(z && a_dev && x && (!y || b_dev), PT_ERR_ARG, "pt_k_lincomb: null argument");
+  PT_REQUIRE(nseg > 0 && len % nseg == 0, PT_ERR_SIZE, "pt_k_lincomb: nseg must divide len");
+  return map(len, FLincomb{(float2*)z, a_dev, (const float2*)x, b_dev, (const float2*)y, len / nseg},
+             (hipStream_t)stream);
+}
+extern "C" int pt_k_dy_update(void* d, const void* grad, const double* beta_dev, int first,
+                              long len, int nseg, pt_stream stream) {
+  PT_REQUIRE(d && grad && (first || beta_dev), PT_ERR_ARG, "pt_k_dy_update: null argument");
+  PT_REQUIRE(nseg > 0 && len % nseg == 0, PT_ERR_SIZE, "pt_k_dy_update: nseg must divide len");
+  return map(len, FDyUpdate{(float2*)d, (const float2*)grad, beta_dev, first, len / nseg},
+             (hipStream_t)stream);
+}
+extern "C" int pt_k_dy_reduce(double* out, const void* grad, const void* grad0, const void* d,
+                              long len, int nseg, pt_stream stream) {
+  PT_REQUIRE(out && grad && grad0 && d, PT_ERR_ARG, "pt_k_dy_reduce: null argument");
+  return seg_reduce<3>(out, len, nseg,
+                       FDyReduce{(const float2*)grad, (const float2*)grad0, (const float2*)d},
+                       (hipStream_t)stream);
+}
+
+extern "C" int pt_k_reg_fwd(void* out, const void* u, int nz, int ny, int nx, pt_stream stream) {
+  PT_REQUIRE(out && u, PT_ERR_ARG, "pt_k_reg_fwd: null argument");
+  return map((long)nz * ny * nx, FRegFwd{(float2*)out, (const float2*)u, nz, ny, nx},
+             (hipStream_t)stream);
+}
+extern "C" int pt_k_reg_adj(void* out, const void* g, int nz, int ny, int nx, pt_stream stream) {
+  PT_REQUIRE(out && g, PT_ERR_ARG, "pt_k_reg_adj: null argument");
+  return map((long)nz * ny * nx, FRegAdj{(float2*)out, (const float2*)g, nz, ny, nx},
+             (hipStream_t)stream);
+}
+extern "C" int pt_k_reg_solve(void* out, const void* u, const void* mu, double tau, double alpha,
+                              int nz, int ny, int nx, pt_stream stream) {
+  PT_REQUIRE(out && u && mu, PT_ERR_ARG, "pt_k_reg_solve: null argument");
+  PT_REQUIRE(tau > 0, PT_ERR_ARG, "pt_k_reg_solve: tau must be positive");
+  return map((long)nz * ny * nx,
+             FRegSolve{(float2*)out, (const float2*)u, (const float2*)mu, (float)(1.0 / tau),
+                       (float)(alpha / tau), nz, ny, nx},
+             (hipStream_t)stream);
+}
+extern "C" int pt_k_exptomo(void* out, const void* psi, double c, double phase, long len,
+                            pt_stream stream) {
+  PT_REQUIRE(out && psi, PT_ERR_ARG, "pt_k_exptomo: null argument");
+  const float2 ph = make_float2((float)std::cos(phase), (float)std::sin(phase));
+  return map(len, FExpTomo{(float2*)out, (const float2*)psi, (float)c, ph}, (hipStream_t)stream);
+}
+extern "C" int pt_k_mlog(void* out, const void* x, long len, pt_stream stream) {
+  PT_REQUIRE(out && x, PT_ERR_ARG, "pt_k_mlog: null argument");
+  return map(len, FMlog{(float2*)out, (const float2*)x}, (hipStream_t)stream);
+}
+
+// max-reduction variant: the stats functor keeps a running max in v[2]; the
+// block sum would add maxima, so use a dedicated kernel.
+namespace pt {
+namespace {
+__global__ __launch_bounds__(RB) void k_takexi_stats(double* __restrict__ partial,
+                                                     FTakexiStats f, long len) {
+  __shared__ double sm[3 * 16];
+  double v[3] = {0.0, 0.0, 0.0};
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < len;
+       i += (long)gridDim.x * blockDim.x)
+    f(i, v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v[0] = wave_sum(v[0]);
+  v[1] = wave_sum(v[1]);
+  for (int off = 32; off > 0; off >>= 1) v[2] = fmax(v[2], __shfl_xor(v[2], off, 64));
+  if (lane == 0) { sm[wid] = v[0]; sm[16 + wid] = v[1]; sm[32 + wid] = v[2]; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0, m = 0;
+    for (int w = 0; w < nw; ++w) { a += sm[w]; b += sm[16 + w]; m = fmax(m, sm[32 + w]); }
+    partial[blockIdx.x * 3 + 0] = a;
+    partial[blockIdx.x * 3 + 1] = b;
+    partial[blockIdx.x * 3 + 2] = m;
+  }
+}
+__global__ void k_takexi_finish(double* out, const double* partial, int nblk) {
+  if (threadIdx.x) return;
+  double a = 0, b = 0, m = 0;
+  for (int i = 0; i < nblk; ++i) {
+    a += partial[i * 3];
+    b += partial[i * 3 + 1];
+    m = fmax(m, partial[i * 3 + 2]);
+  }
+  out[0] = a; out[1] = b; out[2] = m;
+}
+}  // namespace
+}  // namespace pt
+
+extern "C" int pt_k_takexi_stats(double* out, const void* psi, const void* lamd, double rho,
+                                 int ntheta, int nz, int n, int r, pt_stream stream) {
+  PT_REQUIRE(out && psi && lamd, PT_ERR_ARG, "pt_k_takexi_stats: null argument");
+  PT_REQUIRE(rho > 0 && r >= 0 && 2 * r <= n, PT_ERR_ARG, "pt_k_takexi_stats: bad rho or r");
+  hipStream_t st = (hipStream_t)stream;
+  const long len = (long)ntheta * nz * n;
+  const int nblk = (int)std::min<long>(1024, std::max<long>(1, (len + RB * 8 - 1) / (RB * 8)));
+  double* part;
+  int rc = partial_buf((size_t)nblk * 3, &part);
+  if (rc) return rc;
+  FTakexiStats f{(const float2*)psi, (const float2*)lamd, (float)(1.0 / rho), nz, n, r};
+  hipLaunchKernelGGL(k_takexi_stats, dim3(nblk), dim3(RB), 0, st, part, f, len);
+  PT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_takexi_finish, dim3(1), dim3(64), 0, st, out, part, nblk);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_k_takexi(void* xi0, void* K, const void* psi, const void* lamd, double rho,
+                           double pshift, double kc, double kscale, double c1, double coeftomo,
+                           long len, pt_stream stream) {
+  PT_REQUIRE(xi0 && K && psi && lamd, PT_ERR_ARG, "pt_k_takexi: null argument");
+  PT_REQUIRE(rho > 0, PT_ERR_ARG, "pt_k_takexi: rho must be positive");
+  const float2 eph = make_float2((float)std::cos(-pshift), (float)std::sin(-pshift));
+  // K = i*kc*t*kscale ; xi0 = K*(-i*logt/c1)*coeftomo
+  return map(len,
+             FTakexi{(float2*)xi0, (float2*)K, (const float2*)psi, (const float2*)lamd,
+                     (float)rho, eph, (float)(kc * kscale), (float)(coeftomo / c1)},
+             (hipStream_t)stream);
+}
+
+namespace pt {
+namespace {
+__global__ __launch_bounds__(RB) void k_max_angle(double* __restrict__ partial,
+                                                  const float2* __restrict__ x, long len) {
+  __shared__ double sm[16];
+  double m = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < len;
+       i += (long)gridDim.x * blockDim.x) {
+    const float2 a = x[i];
+    m = fmax(m, (double)fabsf(atan2f(a.y, a.x)));
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0) sm[wid] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0;
+    for (int w = 0; w < nw; ++w) r = fmax(r, sm[w]);
+    partial[blockIdx.x] = r;
+  }
+}
+__global__ void k_max_finish(double* out, const double* partial, int nblk) {
+  if (threadIdx.x) return;
+  double r = 0;
+  for (int i = 0; i < nblk; ++i) r = fmax(r, partial[i]);
+  out[0] = r;
+}
+}  // namespace
+}  // namespace pt
+
+extern "C" int pt_k_tomo_resid(void* r, double* out, const void* K, const void* Ru,
+                               const void* xi0, long len, pt_stream stream) {
+  PT_REQUIRE(r && out && K && Ru && xi0, PT_ERR_ARG, "pt_k_tomo_resid: null argument");
+  return seg_reduce<1>(out, len, 1,
+                       FTomoResid{(float2*)r, (const float2*)K, (const float2*)Ru,
+                                  (const float2*)xi0},
+                       (hipStream_t)stream);
+}
+extern "C" int pt_k_tomo_ls(double* out, const void* K, const void* Ru, const void* Rd,
+                            const void* xi0, long len, pt_stream stream) {
+  PT_REQUIRE(out && K && Ru && Rd && xi0, PT_ERR_ARG, "pt_k_tomo_ls: null argument");
+  return seg_reduce<2>(out, len, 1,
+                       FTomoLs{(const float2*)K, (const float2*)Ru, (const float2*)Rd,
+                               (const float2*)xi0},
+                       (hipStream_t)stream);
+}
+extern "C" int pt_k_reg_grad(void* grad, double* out, const void* u, const void* xi1, double tau,
+                             int nz, int ny, int nx, pt_stream stream) {
+  PT_REQUIRE(grad && out && u && xi1, PT_ERR_ARG, "pt_k_reg_grad: null argument");
+  return seg_reduce<1>(out, (long)nz * ny * nx, 1,
+                       FRegGrad{(float2*)grad, (const float2*)u, (const float2*)xi1, (float)tau,
+                                nz, ny, nx},
+                       (hipStream_t)stream);
+}
+extern "C" int pt_k_reg_ls(double* out, const void* u, const void* d, const void* xi1, int nz,
+                           int ny, int nx, pt_stream stream) {
+  PT_REQUIRE(out && u && d && xi1, PT_ERR_ARG, "pt_k_reg_ls: null argument");
+  return seg_reduce<2>(out, (long)nz * ny * nx, 1,
+                       FRegLs{(const float2*)u, (const float2*)d, (const float2*)xi1, nz, ny, nx},
+                       (hipStream_t)stream);
+}
+extern "C" int pt_k_rho_terms(double* out, const void* h, const void* psi, const void* lamd,
+                              const void* d, double rho, long len, int nseg, pt_stream stream) {
+  PT_REQUIRE(out && h && psi && lamd && d, PT_ERR_ARG, "pt_k_rho_terms: null argument");
+  PT_REQUIRE(rho > 0, PT_ERR_ARG, "pt_k_rho_terms: rho must be positive");
+  return seg_reduce<3>(out, len, nseg,
+                       FRhoTerms{(const float2*)h, (const float2*)psi, (const float2*)lamd,
+                                 (const float2*)d, (float)rho},
+                       (hipStream_t)stream);
+}
+extern "C" int pt_k_dot_diff(double* out, const void* a, const void* b, const void* c, long len,
+                             pt_stream stream) {
+  PT_REQUIRE(out && a && b && c, PT_ERR_ARG, "pt_k_dot_diff: null argument");
+  return seg_reduce<2>(out, len, 1, FDotDiff{(const float2*)a, (const float2*)b, (const float2*)c},
+                       (hipStream_t)stream);
+}
+extern "C" int pt_k_tv_norm(double* out, const void* phi, long plane, pt_stream stream) {
+  PT_REQUIRE(out && phi, PT_ERR_ARG, "pt_k_tv_norm: null argument");
+  return seg_reduce<1>(out, plane, 1, FTvNorm{(const float2*)phi, plane}, (hipStream_t)stream);
+}
+extern "C" int pt_k_lincomb3(void* z, double ar, double ai, const void* x, double br, double bi,
+                             const void* y, double cr, double ci, const void* w, long len,
+                             pt_stream stream) {
+  PT_REQUIRE(z && x, PT_ERR_ARG, "pt_k_lincomb3: null argument");
+  return map(len,
+             FLincomb3{(float2*)z, make_float2((float)ar, (float)ai), (const float2*)x,
+                       make_float2((float)br, (float)bi), (const float2*)y,
+                       make_float2((float)cr, (float)ci), (const float2*)w},
+             (hipStream_t)stream);
+}
+extern "C" int pt_k_axpy_seg(void* y, const double* a_dev, const void* x, long len, int nseg,
+                             pt_stream stream) {
+  PT_REQUIRE(y && a_dev && x, PT_ERR_ARG, "pt_k_axpy_seg: null argument");
+  PT_REQUIRE(nseg > 0 && len % nseg == 0, PT_ERR_SIZE, "pt_k_axpy_seg: nseg must divide len");
+  return map(len, FAxpySeg{(float2*)y, a_dev, (const float2*)x, len / nseg}, (hipStream_t)stream);
+}
+extern "C" int pt_k_max_angle(double* out, const void* x, long len, pt_stream stream) {
+  PT_REQUIRE(out && x, PT_ERR_ARG, "pt_k_max_angle: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int nblk = (int)std::min<long>(1024, std::max<long>(1, (len + RB * 8 - 1) / (RB * 8)));
+  double* part;
+  int rc = partial_buf((size_t)nblk, &part);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_max_angle, dim3(nblk), dim3(RB), 0, st, part, (const float2*)x, len);
+  PT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_max_finish, dim3(1), dim3(64), 0, st, out, part, nblk);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+namespace pt {
+namespace {
+struct FIntensity {
+  float* out; const float2* g; float s;
+  __device__ void operator()(long i) const {
+    const float2 a = g[i];
+    const float m = hypotf(a.x, a.y);
+    out[i] = m * m * s;
+  }
+};
+}  // namespace
+}  // namespace pt
+extern "C" int pt_k_intensity(float* out, const void* g, float scale, long len, pt_stream stream) {
+  PT_REQUIRE(out && g, PT_ERR_ARG, "pt_k_intensity: null argument");
+  return map(len, FIntensity{out, (const float2*)g, scale}, (hipStream_t)stream);
+}

@@ -1,0 +1,158 @@
+"""Probe, scan and phantom generators.
+
+``probe`` and ``scanner3`` keep the reference generators' call signatures and
+output formats (``src/ptychotomo/objects.py:4-15`` and ``:18-65``): a P x P
+complex probe, and a scan table of shape [2, ntheta, nscan] float32 with the
+x (column) positions in plane 0 and the y (row / slice) positions in plane 1.
+Their random jitter takes an explicit ``numpy.random.Generator`` instead of
+the global RNG so runs are reproducible.  Plotting (``save=True``) is not
+provided.
+
+The ``gaussian_probe``, ``jittered_raster``, ``ellipsoid_phantom`` and
+``make_theta`` functions build the seeded synthetic inputs of the benchmark
+workloads (Gaussian-amplitude probe with quadratic phase, jittered raster
+scan, random-ellipsoid delta/beta phantom).
+"""
+import numpy as np
+
+PLANCK_CONSTANT = 6.58211928e-19  # [keV*s]
+SPEED_OF_LIGHT = 299792458e+2     # [cm/s]
+
+
+def wavenumber(energy):
+    """2*pi/lambda in 1/cm for an energy in keV (reference solver.py:55-56)."""
+    return 2 * np.pi / (2 * np.pi * PLANCK_CONSTANT * SPEED_OF_LIGHT / energy)
+
+
+def probe(size, maxint=1, rin=0.8, rout=1):
+    """Ring probe with a linear taper and 0.2 rad phase (objects.py:4-15)."""
+    r, c = np.mgrid[:size, :size] + 0.5
+    rs = np.sqrt((r - size / 2) ** 2 + (c - size / 2) ** 2)
+    rmax = np.sqrt(2) * 0.5 * rout * rs.max() + 1.0
+    rmin = np.sqrt(2) * 0.5 * rin * rs.max()
+    img = np.zeros((size, size), dtype='float32')
+    img[rs < rmin] = 1.0
+    img[rs > rmax] = 0.0
+    zone = np.logical_and(rs > rmin, rs < rmax)
+    img[zone] = np.divide(rmax - rs[zone], rmax - rmin)
+    return (np.sqrt(maxint) * img * np.exp(1j * 0.2 * img)).astype(np.complex64)
+
+
+def scanner3(theta, shape, sx, sy, psize, spiral=0, randscan=False, rng=None):
+    """Raster scan table [2, ntheta, nscan] (objects.py:18-38, 154-157).
+
+    Positions run over [0, shape[1]-psize] in x and [0, shape[0]-psize] in y
+    with steps (sx, sy); ``randscan`` adds a per-angle offset of up to half a
+    step and a per-position jitter of up to 5 % of a step; values whose
+    rounded position falls outside the object are clamped as in the reference.
+    """
+    rng = np.random.default_rng() if rng is None else rng
+    scx, scy = np.meshgrid(np.arange(0, shape[1] - psize + 1, sx),
+                           np.arange(0, shape[0] - psize + 1, sy))
+    shapescan = np.size(scx)
+    scanax = -1 + np.zeros([len(theta), shapescan], dtype='float32')
+    scanay = -1 + np.zeros([len(theta), shapescan], dtype='float32')
+    a = spiral
+    for m in range(len(theta)):
+        scanax[m] = np.ndarray.flatten(scx) + np.mod(a, sx)
+        scanay[m] = np.ndarray.flatten(scy)
+        a += spiral
+        if randscan:
+            scanax[m] += sx * (rng.random(1) - 0.5) * 1
+            scanay[m] += sy * (rng.random(1) - 0.5) * 1
+            scanax[m] += sx * (rng.random(shapescan) - 0.5) * 0.1
+            scanay[m] += sy * (rng.random(shapescan) - 0.5) * 0.1
+    scanax[np.where(np.round(scanax) < 0)] = 0
+    scanay[np.where(np.round(scanay) < 0)] = 0
+    scanax[np.where(np.round(scanax) > shape[1] - psize)] = shape[1] - psize - 1
+    scanay[np.where(np.round(scanay) > shape[0] - psize)] = shape[0] - psize - 1
+    scan = np.zeros([2, len(theta), shapescan], dtype='float32', order='C')
+    scan[0] = scanax
+    scan[1] = scanay
+    return scan
+
+
+def make_theta(ntheta, endpoint=False):
+    """Rotation angles on [0, pi), float32."""
+    return np.linspace(0, np.pi, ntheta, endpoint=endpoint).astype(np.float32)
+
+
+def gaussian_probe(size, sigma, edge_phase=0.5):
+    """P x P probe: amplitude exp(-r^2/(2 sigma^2)), phase edge_phase*(r/(P/2))^2.
+
+    r is measured from pixel (P/2, P/2).  A probe mirror-symmetric about the
+    array centre ((P-1)/2) has exactly-zero Nyquist far-field rows for a
+    constant object (the ADMM initial guess psi = 1), where the Gaussian
+    residual's phase and the Poisson residual are decided by FFT rounding;
+    centring on a pixel avoids those symmetry-forced zeros.  Peak amplitude 1.
+    """
+    c = size / 2.0
+    yy, xx = np.mgrid[:size, :size]
+    r2 = (yy - c) ** 2 + (xx - c) ** 2
+    amp = np.exp(-r2 / (2.0 * sigma ** 2))
+    phase = edge_phase * r2 / (size / 2.0) ** 2
+    return (amp * np.exp(1j * phase)).astype(np.complex64)
+
+
+def jittered_raster(ntheta, nz, n, psize, step, jitter, rng):
+    """Raster with ``step`` over [0, n-psize] x [0, nz-psize], plus uniform
+    jitter in [-jitter, jitter] per position and angle, clamped into range.
+    Returns [2, ntheta, nscan] float32 (x first)."""
+    xs = np.arange(0, n - psize + 1, step)
+    ys = np.arange(0, nz - psize + 1, step)
+    gx, gy = np.meshgrid(xs, ys)
+    gx = gx.ravel().astype(np.float64)
+    gy = gy.ravel().astype(np.float64)
+    S = gx.size
+    scan = np.empty((2, ntheta, S), np.float32)
+    jx = rng.uniform(-jitter, jitter, (ntheta, S))
+    jy = rng.uniform(-jitter, jitter, (ntheta, S))
+    scan[0] = np.clip(gx[None, :] + jx, 0, n - psize)
+    scan[1] = np.clip(gy[None, :] + jy, 0, nz - psize)
+    return scan
+
+
+def ellipsoid_phantom(nz, n, nellipse, rng, voxelsize=1e-6, energy=8.8,
+                      semi=(4.0, 20.0), max_phase=1.0):
+    """Complex object u = delta + i*beta of shape [nz, n, n], complex64.
+
+    ``nellipse`` ellipsoids with semi-axes uniform in ``semi`` voxels, uniform
+    random orientations, centres uniform inside the inscribed cylinder (kept
+    one bounding radius away from the walls), additive delta uniform in
+    [0, 1].  delta is scaled so that the larger of the 0- and 90-degree
+    projected phases ``voxelsize*k*sum(delta)`` peaks at ``max_phase`` rad;
+    beta = delta/10.
+    """
+    delta = np.zeros((nz, n, n), np.float32)
+    zz = np.arange(nz, dtype=np.float32)
+    yy = np.arange(n, dtype=np.float32)
+    xx = np.arange(n, dtype=np.float32)
+    for _ in range(nellipse):
+        a = rng.uniform(semi[0], semi[1], 3)
+        rmax = float(min(a.max(), n / 2 - 2, nz / 2 - 2))
+        rad = rng.uniform(0, 1) ** 0.5 * max(n / 2 - rmax - 1, 0)
+        ang = rng.uniform(0, 2 * np.pi)
+        cx = n / 2 + rad * np.cos(ang)
+        cy = n / 2 + rad * np.sin(ang)
+        cz = rng.uniform(rmax, max(nz - rmax, rmax + 1e-3))
+        q = rng.standard_normal(4)
+        q /= np.linalg.norm(q)
+        w, x, y, z = q
+        Rm = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                       [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                       [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        val = rng.uniform(0, 1)
+        z0, z1 = int(max(0, np.floor(cz - rmax))), int(min(nz, np.ceil(cz + rmax) + 1))
+        y0, y1 = int(max(0, np.floor(cy - rmax))), int(min(n, np.ceil(cy + rmax) + 1))
+        x0, x1 = int(max(0, np.floor(cx - rmax))), int(min(n, np.ceil(cx + rmax) + 1))
+        if z1 <= z0 or y1 <= y0 or x1 <= x0:
+            continue
+        Z, Y, X = np.meshgrid(zz[z0:z1] - cz, yy[y0:y1] - cy, xx[x0:x1] - cx, indexing='ij')
+        P = np.stack([X, Y, Z], -1) @ Rm  # body-frame coordinates
+        inside = ((P[..., 0] / a[0]) ** 2 + (P[..., 1] / a[1]) ** 2 + (P[..., 2] / a[2]) ** 2) <= 1.0
+        delta[z0:z1, y0:y1, x0:x1] += np.float32(val) * inside
+    k = wavenumber(energy)
+    proj = max(delta.sum(axis=2).max(), delta.sum(axis=1).max())
+    if proj > 0:
+        delta *= np.float32(max_phase / (voxelsize * k * proj))
+    return (delta + 1j * (delta / 10)).astype(np.complex64)

@@ -1,0 +1,50 @@
+"""Drop-in for the reference SWIG class ``ptychofft``
+(src/include/ptychofft.cuh:3-36, src/cuda/ptychofft.i:60-94).
+
+Same constructor (including the unused ``Ntheta0``) and methods; buffers
+are raw device addresses, output first, unnormalised.  Work is enqueued on
+PyTorch's current HIP stream.
+"""
+import ctypes
+
+from . import _lib
+from ._dev import stream
+
+
+class ptychofft:
+    def __init__(self, Ntheta, Nz, N, Ntheta0, Nscan, detx, dety, Nprb):
+        h = ctypes.c_void_p()
+        _lib.call("pt_ptycho_create", int(Ntheta), int(Nz), int(N), int(Ntheta0), int(Nscan),
+                  int(detx), int(dety), int(Nprb), ctypes.byref(h))
+        self._h = h
+        self.Ntheta, self.Nz, self.N, self.Nscan = int(Ntheta), int(Nz), int(N), int(Nscan)
+        self.detx, self.dety, self.Nprb = int(detx), int(dety), int(Nprb)
+
+    def setobj(self, scan, prb):
+        _lib.call("pt_ptycho_set_scan_probe", self._h, ctypes.c_void_p(int(scan)),
+                  ctypes.c_void_p(int(prb)), stream())
+
+    def fwd(self, g, f):
+        _lib.call("pt_ptycho_fwd", self._h, ctypes.c_void_p(int(g)), ctypes.c_void_p(int(f)), stream())
+
+    def adj(self, f, g):
+        _lib.call("pt_ptycho_adj", self._h, ctypes.c_void_p(int(f)), ctypes.c_void_p(int(g)), stream())
+
+    def fwd_scaled(self, g, f, scale):
+        _lib.call("pt_ptycho_fwd_scaled", self._h, ctypes.c_void_p(int(g)), ctypes.c_void_p(int(f)),
+                  float(scale), stream())
+
+    def adj_scaled(self, f, g, scale):
+        _lib.call("pt_ptycho_adj_scaled", self._h, ctypes.c_void_p(int(f)), ctypes.c_void_p(int(g)),
+                  float(scale), stream())
+
+    def free(self):
+        if getattr(self, "_h", None):
+            _lib.lib.pt_ptycho_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,264 @@
+"""GPU Solver methods against the NumPy restatement of the reference solver
+(oracle.solver.Solver) on identical seeded inputs."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+from oracle import solver as osolver  # noqa: E402
+from ptychotomo_amd import objects  # noqa: E402
+
+
+def rel(a, b):
+    if isinstance(a, torch.Tensor):
+        a = a.cpu().numpy()
+    if isinstance(b, torch.Tensor):
+        b = b.cpu().numpy()
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def crandn(rng, shape, s=1.0):
+    return (s * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))).astype(np.complex64)
+
+
+CFG = dict(n=32, nz=32, ntheta=24, nprb=8, det=8, step=4, ptheta=8, pnz=16, nell=6)
+
+
+@pytest.fixture(scope="module")
+def setup():
+    from ptychotomo_amd.solver import Solver
+    c = CFG
+    rng = np.random.default_rng(0)
+    theta = objects.make_theta(c["ntheta"])
+    prb = objects.gaussian_probe(c["nprb"], c["nprb"] / 4)
+    scan = objects.jittered_raster(c["ntheta"], c["nz"], c["n"], c["nprb"], c["step"], 1.0,
+                                   np.random.default_rng(1))
+    obj = objects.ellipsoid_phantom(c["nz"], c["n"], c["nell"], rng, semi=(3, 8))
+    args = (prb, scan, theta, [c["det"], c["det"]], 1e-6, 8.8, c["ntheta"], c["nz"], c["n"],
+            c["ptheta"], c["pnz"])
+    # scale the probe to 1e2 photons at the brightest detector pixel: the
+    # photon-limited regime where the reference iteration is well conditioned
+    # (its FP32 and FP64 restatements take identical line-search decisions)
+    d0 = osolver.Solver(*args).fwd_ptycho_batch(osolver.Solver(*args).exptomo(
+        osolver.Solver(*args).fwd_tomo_batch(obj)))
+    prb = (prb * np.float32(np.sqrt(1e2 / d0.max()))).astype(np.complex64)
+    args = (prb,) + args[1:]
+    ref = osolver.Solver(*args)
+    slv = Solver(*args)
+    slv.ref64 = osolver.Solver(*args, dtype=np.complex128)
+    data = ref.fwd_ptycho_batch(ref.exptomo(ref.fwd_tomo_batch(obj)))
+    data = np.random.default_rng(2).poisson(data).astype(np.float32)
+    return ref, slv, obj, data, rng
+
+
+def test_operators(setup):
+    ref, slv, obj, data, rng = setup
+    c = CFG
+    print("operators", flush=True)
+    e = rel(slv.fwd_tomo_batch(obj), ref.fwd_tomo_batch(obj))
+    assert e < 1e-5, e
+    g = crandn(rng, (c["ntheta"], c["nz"], c["n"]))
+    e = rel(slv.adj_tomo_batch(g), ref.adj_tomo_batch(g))
+    assert e < 1e-5, e
+    u = crandn(rng, (c["pnz"], c["n"], c["n"]))
+    e = rel(slv.fwd_tomo(u), ref.fwd_tomo(u))
+    assert e < 1e-5, e
+    psi = np.exp(1j * crandn(rng, (c["ntheta"], c["nz"], c["n"]), 0.3)).astype(np.complex64)
+    # chunk operators on chunk 1
+    ids = np.arange(c["ptheta"], 2 * c["ptheta"])
+    ref.setobj(ids)
+    sc = torch.from_numpy(np.ascontiguousarray(ref.scan[:, ids])).cuda()
+    slv.cl_ptycho.setobj(sc.data_ptr(), slv.prb.data_ptr())
+    e = rel(slv.fwd_ptycho(psi[ids]), ref.fwd_ptycho(psi[ids]))
+    assert e < 1e-5, e
+    far = crandn(rng, (c["ptheta"], ref.nscan, c["det"], c["det"]))
+    e = rel(slv.adj_ptycho(far), ref.adj_ptycho(far))
+    assert e < 1e-5, e
+    e = rel(slv.fwd_ptycho_batch(psi), ref.fwd_ptycho_batch(psi))
+    assert e < 1e-5, e
+
+
+def test_elementwise(setup):
+    ref, slv, obj, data, rng = setup
+    c = CFG
+    print("elementwise", flush=True)
+    u = crandn(rng, (c["nz"], c["n"], c["n"]))
+    e = rel(slv.fwd_reg(u), ref.fwd_reg(u))
+    assert e < 1e-6, e
+    g = crandn(rng, (3, c["nz"], c["n"], c["n"]))
+    e = rel(slv.adj_reg(g), ref.adj_reg(g))
+    assert e < 1e-6, e
+    mu = crandn(rng, (3, c["nz"], c["n"], c["n"]), 0.3)
+    e = rel(slv.solve_reg(u, mu, 0.7, 0.9), ref.solve_reg(u, mu, 0.7, 0.9))
+    assert e < 1e-6, e
+    x = crandn(rng, (c["ntheta"], c["nz"], c["n"]), 1e-3)
+    e = rel(slv.exptomo(x), ref.exptomo(x))
+    assert e < 1e-5, e
+    x[0, 0, :3] = 0
+    e = rel(slv.mlog(x), ref.mlog(x))
+    assert e < 1e-6, e
+    psi = np.exp(1j * crandn(rng, (c["ntheta"], c["nz"], c["n"]), 0.2)).astype(np.complex64)
+    lamd = crandn(rng, psi.shape, 0.01)
+    phi = crandn(rng, (3, c["nz"], c["n"], c["n"]), 0.1)
+    a = slv.takexi(psi, phi, lamd, mu, 0.5, 0.25)
+    b = ref.takexi(psi, phi, lamd, mu, 0.5, 0.25)
+    assert abs(a[3] - b[3]) < 1e-6
+    for k in range(3):
+        e = rel(a[k], b[k])
+        assert e < 1e-5, (k, e)
+
+
+def test_cg_tomo(setup):
+    ref, slv, obj, data, rng = setup
+    c = CFG
+    print("cg_tomo", flush=True)
+    psi = ref.exptomo(ref.fwd_tomo_batch(obj))
+    lamd = np.zeros_like(psi)
+    phi = np.zeros((3, c["nz"], c["n"], c["n"]), np.complex64)
+    mu = np.zeros_like(phi)
+    xi0, xi1, K, _ = ref.takexi(psi, phi, lamd, mu, 0.5, 0.5)
+    u0 = np.zeros((c["nz"], c["n"], c["n"]), np.complex64)
+    ua = slv.cg_tomo(xi0, xi1, K, u0, 0.5, 0.5, 4)
+    ub = ref.cg_tomo(xi0, xi1, K, u0, 0.5, 0.5, 4)
+    print("gammas", slv._last_tomo_gammas, ref._last_tomo_gammas)
+    assert slv._last_tomo_gammas == ref._last_tomo_gammas
+    e = rel(ua, ub)
+    assert e < 1e-4, e
+
+
+@pytest.mark.parametrize("model", ["poisson", "gaussian"])
+def test_cg_ptycho(setup, model):
+    ref, slv, obj, data, rng = setup
+    c = CFG
+    print("cg_ptycho", model, flush=True)
+    psi0 = np.ones((c["ntheta"], c["nz"], c["n"]), np.complex64)
+    h = ref.exptomo(ref.fwd_tomo_batch(obj * np.float32(0.5)))
+    lamd = crandn(rng, psi0.shape, 1e-3)
+    pa = slv.cg_ptycho_batch(data, psi0, h, lamd, 0.5, 6, model)
+    pb = ref.cg_ptycho_batch(data, psi0, h, lamd, 0.5, 6, model)
+    pc = slv.ref64.cg_ptycho_batch(data, psi0, h, lamd, 0.5, 6, model)
+    print("gammas", slv._chunk_gammas, ref._chunk_gammas, slv.ref64._chunk_gammas)
+    assert slv._chunk_gammas == slv.ref64._chunk_gammas
+    # The FP32 restatement itself drifts from its FP64 twin (Dai-Yuan
+    # denominators cancel); the device result must be at least as close to
+    # the FP64 twin as the FP32 restatement is (or within 1e-5).
+    e32, e64, ref_drift = rel(pa, pb), rel(pa, pc), rel(pb, pc)
+    print(f"gpu-fp32 {e32:.2e} gpu-fp64 {e64:.2e} fp32-fp64 {ref_drift:.2e}")
+    assert e64 <= max(1.5 * ref_drift, 1e-5), (e64, ref_drift)
+
+
+def test_penalty_lagr(setup):
+    ref, slv, obj, data, rng = setup
+    c = CFG
+    print("penalty/lagr", flush=True)
+    sh = (c["ntheta"], c["nz"], c["n"])
+    psi = np.exp(1j * crandn(rng, sh, 0.2)).astype(np.complex64)
+    h = (psi + crandn(rng, sh, 0.01)).astype(np.complex64)
+    h0 = (psi + crandn(rng, sh, 0.02)).astype(np.complex64)
+    phi = crandn(rng, (3, c["nz"], c["n"], c["n"]), 0.1)
+    e = (phi + crandn(rng, phi.shape, 0.01)).astype(np.complex64)
+    e0 = (phi + crandn(rng, phi.shape, 0.05)).astype(np.complex64)
+    lamd = crandn(rng, sh, 0.1)
+    mu = crandn(rng, phi.shape, 0.1)
+    T = lambda x: torch.from_numpy(x).cuda()
+    a = slv.update_penalty(T(psi), T(h), T(h0), T(phi), T(e), T(e0), 0.5, 0.5)
+    b = ref.update_penalty(psi, h, h0, phi, e, e0, 0.5, 0.5)
+    assert a == b
+    for model in ["poisson", "gaussian"]:
+        la = slv.take_lagr(T(psi), T(phi), data, T(h), T(e), T(lamd), T(mu), 3e-7, 0.5, 0.5, model)
+        lb = ref.take_lagr(psi, phi, data, h, e, lamd, mu, 3e-7, 0.5, 0.5, model)
+        err = np.abs(la - lb) / np.maximum(np.abs(lb), 1e-30)
+        print(model, la, lb)
+        assert np.all(err < 1e-4), err
+
+
+def test_admm_trajectory(setup):
+    ref, slv, obj, data, rng = setup
+    c = CFG
+    print("admm", flush=True)
+    sh = (c["ntheta"], c["nz"], c["n"])
+    sh3 = (3, c["nz"], c["n"], c["n"])
+    init = dict(h=np.ones(sh, np.complex64), e=np.zeros(sh3, np.complex64),
+                psi=np.ones(sh, np.complex64), phi=np.zeros(sh3, np.complex64),
+                lamd=np.zeros(sh, np.complex64), mu=np.zeros(sh3, np.complex64),
+                u=np.zeros((c["nz"], c["n"], c["n"]), np.complex64))
+    niter = 4
+    ua, pa = slv.admm(data, init["h"], init["e"], init["psi"], init["phi"], init["lamd"],
+                      init["mu"], init["u"], 3e-7, 4, 4, niter, "poisson", lagr_every=1,
+                      verbose=False)
+    ub, pb = ref.admm(data, init["h"], init["e"], init["psi"], init["phi"], init["lamd"],
+                      init["mu"], init["u"], 3e-7, 4, 4, niter, "poisson", lagr_every=1)
+    r64 = slv.ref64
+    uc, pc = r64.admm(data, init["h"], init["e"], init["psi"], init["phi"], init["lamd"],
+                      init["mu"], init["u"], 3e-7, 4, 4, niter, "poisson", lagr_every=1)
+    assert len(slv.history) == len(r64.history) == niter
+    for ra, rb, rc in zip(slv.history, ref.history, r64.history):
+        print(ra["m"], ra["rho"], rb["rho"], ra["tau"], rb["tau"], ra["lagr"][6], rb["lagr"][6],
+              rc["lagr"][6])
+        assert ra["rho"] == rc["rho"] and ra["tau"] == rc["tau"]
+        assert ra["tomo_gammas"] == rc["tomo_gammas"]
+        assert ra["ptycho_gammas"] == rc["ptycho_gammas"]
+        la, lb, lc = ra["lagr"][6], rb["lagr"][6], rc["lagr"][6]
+        assert abs(la - lc) <= max(1.5 * abs(lb - lc), 1e-4 * abs(lc)), (la, lb, lc)
+    eu, ep = rel(ua, uc), rel(pa, pc)
+    du, dp = rel(ub, uc), rel(pb, pc)
+    print(f"u: gpu-fp64 {eu:.2e} fp32-fp64 {du:.2e}; psi: gpu-fp64 {ep:.2e} fp32-fp64 {dp:.2e}")
+    assert eu <= max(1.5 * du, 1e-4) and ep <= max(1.5 * dp, 1e-4)
+
+
+@pytest.mark.parametrize("model", ["poisson", "gaussian"])
+def test_ptycho_grad_minf(setup, model):
+    """Fused gradient and fused line-search data term against the unfused
+    oracle chain on one chunk."""
+    from ptychotomo_amd import _lib
+    from ptychotomo_amd._dev import ptr, stream
+    ref, slv, obj, data, rng = setup
+    c = CFG
+    print("grad/minf", model, flush=True)
+    sh = (c["ntheta"], c["nz"], c["n"])
+    psi = np.exp(1j * crandn(rng, sh, 0.2)).astype(np.complex64)
+    h = (psi + crandn(rng, sh, 0.05)).astype(np.complex64)
+    lamd = crandn(rng, sh, 0.01)
+    dirn = crandn(rng, sh, 0.01)
+    rho = 0.5
+    T = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    dd, psid, hd, ld, dird = T(data), T(psi), T(h), T(lamd), T(dirn)  # keep alive
+    grad = torch.empty(sh, dtype=torch.complex64, device="cuda")
+    _lib.call("pt_ptycho_grad", slv._ptycho._h, ptr(grad), ptr(psid), ptr(dd), ptr(hd),
+              ptr(ld), rho, float(slv.coefptycho), float(slv.coefdata),
+              _lib.PT_MODEL[model], stream())
+    gam = np.array([0.0, 0.3, 1.0], np.float64)
+    ng = len(gam)
+    sums = torch.zeros(slv.nchunk * ng, dtype=torch.float64, device="cuda")
+    gdev = T(np.tile(gam, slv.nchunk))
+    _lib.call("pt_ptycho_minf", slv._ptycho._h, ptr(sums), ptr(psid), ptr(dird), ptr(dd),
+              ptr(gdev), ng, c["ptheta"], float(slv.coefptycho), float(slv.coefdata),
+              _lib.PT_MODEL[model], 0, stream())
+    g = grad.cpu().numpy()
+    s = sums.cpu().numpy().reshape(slv.nchunk, ng)
+    for k in range(slv.nchunk):
+        ids = np.arange(k * c["ptheta"], (k + 1) * c["ptheta"])
+        ref.setobj(ids)
+        datap = data[ids] * ref.coefdata
+        fpsi = ref.fwd_ptycho(psi[ids])
+        if model == "poisson":
+            gr = ref.adj_ptycho(fpsi - datap * fpsi / (np.abs(fpsi) ** 2 + 1e-32))
+        else:
+            gr = ref.adj_ptycho(fpsi - np.sqrt(datap) * np.exp(1j * np.angle(fpsi)))
+        gr = gr - rho * (h[ids] - psi[ids] + lamd[ids] / rho)
+        e = rel(g[ids], gr)
+        print("chunk", k, "grad rel", e)
+        assert e < 1e-5, e
+        fd = ref.fwd_ptycho(dirn[ids])
+        for j, gm in enumerate(gam):
+            a = (fpsi + np.float32(gm) * fd).astype(np.complex128)
+            d64 = datap.astype(np.float64)
+            if model == "poisson":
+                fr = np.sum(np.abs(a) ** 2 - 2 * d64 * np.log(np.maximum(np.abs(a), 1e-32)))
+            else:
+                fr = np.sum((np.abs(a) - np.sqrt(d64)) ** 2)
+            print("chunk", k, "gamma", gm, s[k, j], fr)
+            assert abs(s[k, j] - fr) <= 1e-5 * abs(fr) + 1e-3
