@@ -1,0 +1,254 @@
+#!/usr/bin/env python
+"""Benchmark: plain ADMM iterations per second for joint ptycho-tomography on
+one MI355X (256^3 object, 384 angles, 16x16 probe, scan step 4, Poisson
+data at 1e3 photons; piter = titer = 4).
+
+A step is one ADMM outer iteration (cg_ptycho over all angle chunks,
+takexi, cg_tomo, solve_reg, h/e/dual/penalty updates, Lagrangian every 10
+iterations as in the reference), with all inputs resident in HBM.  W warm-up
+iterations (including the two piter+32 iterations of the reference
+schedule) precede K timed iterations.
+
+Prints one JSON line with the headline number, the roofline of the dominant
+kernel (HIP events around each of its launches inside the timed region) and
+the CPU baseline (the NumPy restatement of the reference operators on a
+bounded sample, run on this host's cores).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(tag, slv):
+    """Algorithmic HBM bytes of one launch of kernel `tag` (each input and
+    output counted once)."""
+    n, nz, nt = slv.n, slv.nz, slv.ntheta
+    N = 3 * n // 2
+    S, D = slv.nscan, slv.ndety
+    vol_t = 8 * nt * nz * n            # psi-sized complex64 array
+    if tag == "tomo_gather":            # 2N x 2N grid in, cropped projections out
+        return 8 * (2 * N) ** 2 * nz + vol_t
+    if tag == "tomo_scatter":           # polar samples in, grid out
+        return 8 * nt * N * nz + 8 * (2 * N) ** 2 * nz
+    if tag == "ptycho_grad_tile":       # psi in, data in, one psi-sized result out
+        return vol_t + 4 * nt * S * D * D + vol_t
+    if tag == "ptycho_minf":            # psi, d and data in
+        return 2 * vol_t + 4 * nt * S * D * D
+    if tag == "tomo_col_fwd":           # N x 2N in, 2N x 2N out
+        return 8 * N * 2 * N * nz + 8 * (2 * N) ** 2 * nz
+    if tag == "tomo_row_fwd":           # object in, N x 2N out
+        return 8 * n * n * nz + 8 * N * 2 * N * nz
+    if tag == "tomo_col_adj":           # 2N x 2N in, n x 2N out
+        return 8 * (2 * N) ** 2 * nz + 8 * n * 2 * N * nz
+    if tag == "tomo_row_adj":           # n x 2N in, object out
+        return 8 * n * 2 * N * nz + 8 * n * n * nz
+    if tag == "tomo_fold":
+        return 8 * (2 * N) ** 2 * nz * 2
+    return None
+
+
+TAGS = ["tomo_gather", "tomo_scatter", "tomo_fold", "tomo_row_fwd", "tomo_col_fwd",
+        "tomo_adj_line", "tomo_col_adj", "tomo_row_adj", "ptycho_grad_tile", "ptycho_fold",
+        "ptycho_minf"]
+
+
+def prof_run(lib, tag, fn):
+    lib.pt_prof_start(tag.encode())
+    fn()
+    tot = ctypes.c_double()
+    cnt = ctypes.c_long()
+    lib.pt_prof_stop(ctypes.byref(tot), ctypes.byref(cnt))
+    return tot.value, cnt.value
+
+
+def cpu_baseline(w, budget_s=30.0):
+    """NumPy/SciPy restatement of the reference operators (oracle) timed on a
+    bounded sample of the workload and scaled to one reference ADMM
+    iteration: 9 USFFT forward + 4 adjoint per iteration (cg_tomo with titer
+    = 4 plus the h update), and per ptychography CG iteration 2 forward, 1
+    adjoint and 3 far-field residual/functional passes (solver.py:206-242)."""
+    import scipy.fft  # noqa: F401
+    from oracle import ptycho as optycho
+    from oracle import usfft as ousfft
+    from ptychotomo_amd import synthetic
+    cores = len(os.sched_getaffinity(0))
+    theta, prb, scan, obj = synthetic.make_inputs(w)
+    rng = np.random.default_rng(0)
+    pz = 2
+    t0 = time.perf_counter()
+    ousfft.interp_matrix(theta, w.n)  # geometry set-up, not counted
+    t_setup = time.perf_counter() - t0
+    u = obj[:pz]
+    t0 = time.perf_counter()
+    g = ousfft.fwd(u, theta, workers=cores)
+    t_tf = (time.perf_counter() - t0) * (w.nz / pz)
+    t0 = time.perf_counter()
+    ousfft.adj(g, theta, workers=cores)
+    t_ta = (time.perf_counter() - t0) * (w.nz / pz)
+    na = 8
+    psi = np.exp(1j * 1e-2 * rng.standard_normal((na, w.nz, w.n))).astype(np.complex64)
+    sc = scan[:, :na]
+    t0 = time.perf_counter()
+    far = optycho.fwd(psi, prb, sc, w.det, w.det)
+    t_pf = (time.perf_counter() - t0) * (w.ntheta / na)
+    t0 = time.perf_counter()
+    optycho.adj(far, prb, sc, w.nz, w.n)
+    t_pa = (time.perf_counter() - t0) * (w.ntheta / na)
+    d = np.abs(far) ** 2
+    t0 = time.perf_counter()
+    r = far - d * far / (np.abs(far) ** 2 + 1e-32)
+    f = np.sum(np.abs(far) ** 2 - 2 * d * np.log(np.maximum(np.abs(far), 1e-32)))
+    del r, f
+    t_pr = (time.perf_counter() - t0) * (w.ntheta / na)
+    per_iter = 9 * t_tf + 4 * t_ta + w.piter * (2 * t_pf + t_pa + 3 * t_pr)
+    return {"value": 1.0 / per_iter, "unit": "ADMM iters/s", "cores": cores, "kind": "port",
+            "sample": (f"oracle NumPy/SciPy FP32 restatement at {w.n}^3/{w.ntheta} angles: USFFT "
+                       f"fwd+adj on {pz} slices (x{w.nz // pz}), ptychography fwd+adj+residual "
+                       f"on {na} angles (x{w.ntheta // na}), composed to one reference ADMM "
+                       f"iteration (9 USFFT fwd, 4 adj, piter x (2 fwd, 1 adj, 3 residual "
+                       f"passes)); interpolation-matrix set-up {t_setup:.1f}s excluded"),
+            "seconds_per_iter": per_iter,
+            "op_seconds": {"usfft_fwd": t_tf, "usfft_adj": t_ta, "ptycho_fwd": t_pf,
+                           "ptycho_adj": t_pa, "ptycho_residual": t_pr}}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="prod256")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--breakdown", action="store_true", help="per-kernel time per iteration")
+    ap.add_argument("--tol-run", type=int, default=0,
+                    help="also run N plain iterations for time-to-Lagrangian-tolerance")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print(json.dumps({"error": "this build targets one GPU; multi-GPU runs are not built "
+                                   "(see DESIGN.md for how the path would shard)",
+                          "n_gpus": max(args.gpus, world)}))
+        sys.exit(2)
+
+    import torch
+    from ptychotomo_amd import _lib, synthetic
+
+    w = synthetic.CONFIGS[args.workload]
+    log = lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    log(f"workload {args.workload}: generating data")
+    t0 = time.perf_counter()
+    slv, data, obj = synthetic.make_workload(w)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    st0 = synthetic.initial_state(w)
+    run = dict(alpha=w.alpha, piter=w.piter, titer=w.titer, model=w.model, verbose=False)
+
+    log(f"data ready in {t_gen:.1f}s; warm-up {args.warmup} iterations")
+    # warm-up: includes the two piter+32 iterations of the reference schedule
+    slv.admm(data, st0["h"], st0["e"], st0["psi"], st0["phi"], st0["lamd"], st0["mu"],
+             st0["u"], niter=args.warmup, lagr_every=10, **run)
+    torch.cuda.synchronize()
+
+    # dominant kernel from a per-kernel breakdown over one (untimed) iteration
+    lib = _lib.lib
+    saved = dict(slv.state)
+    breakdown = {}
+    for tag in (TAGS if args.breakdown else ["tomo_gather", "tomo_scatter", "ptycho_grad_tile",
+                                              "ptycho_minf"]):
+        slv.state = dict(saved)
+        ms, cnt = prof_run(lib, tag, lambda: slv.admm_resume(data, niter=1, lagr_every=10, **run))
+        breakdown[tag] = {"ms_per_iter": ms, "launches": cnt}
+    slv.state = dict(saved)
+    dom = max(breakdown, key=lambda k: breakdown[k]["ms_per_iter"])
+    log(f"breakdown {breakdown}; timing {args.steps} iterations, roofline kernel {dom}")
+
+    # timed region
+    torch.cuda.synchronize()
+    lib.pt_prof_start(dom.encode())
+    t0 = time.perf_counter()
+    slv.admm_resume(data, niter=args.steps, lagr_every=10, **run)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    tot = ctypes.c_double()
+    cnt = ctypes.c_long()
+    lib.pt_prof_stop(ctypes.byref(tot), ctypes.byref(cnt))
+    avg_ms = tot.value / max(cnt.value, 1)
+    nbytes = kernel_bytes(dom, slv)
+    achieved = nbytes / (avg_ms * 1e-3) / 1e9 if nbytes else None
+    traffic = None
+    tf = os.path.join(HERE, "profiles", f"{dom}_{args.workload}_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "ADMM iters/s + time-to-Lagrangian-tol, 256^3, plain vs warm-start, 1 MI355X",
+        "value": args.steps / dt,
+        "unit": "ADMM iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 (fp32 complex), fp64 reductions",
+        "data": "synthetic (seeded ellipsoid phantom, Poisson counts)",
+        "config": synthetic.CONFIGS[args.workload].describe() | {"parallelism": "single GPU"},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": traffic, "algorithmic_bytes": nbytes, "avg_launch_ms": avg_ms,
+                     "launches": cnt.value},
+        "breakdown_ms_per_iter": {k: round(v["ms_per_iter"], 3) for k, v in breakdown.items()},
+        "datagen_s": round(t_gen, 2),
+    }
+    lag = [r.get("lagr") for r in slv.history if r.get("lagr")]
+    if lag:
+        out["lagr_last"] = lag[-1]
+    if args.tol_run:
+        out["time_to_tol"] = time_to_tol(slv, data, w, run, args.tol_run)
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(out), flush=True)
+
+
+def time_to_tol(slv, data, w, run, niter, tol=1e-3):
+    """Plain ADMM from the reference initial guess for `niter` iterations with
+    the Lagrangian every iteration; time until lagr[6] stays within `tol`
+    (relative) of its final value."""
+    import torch
+    from ptychotomo_amd import synthetic
+    st = synthetic.initial_state(w)
+    stamps = []
+
+    def cb(m, s):
+        torch.cuda.synchronize()
+        stamps.append(time.perf_counter())
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    slv.admm(data, st["h"], st["e"], st["psi"], st["phi"], st["lamd"], st["mu"], st["u"],
+             niter=niter, lagr_every=1, callback=cb, **run)
+    lag = np.array([r["lagr"][6] for r in slv.history])
+    final = lag[-1]
+    ok = np.abs(lag - final) <= tol * abs(final)
+    k = len(ok) - 1
+    while k > 0 and ok[k - 1]:
+        k -= 1
+    return {"plain_s": stamps[k] - t0, "iterations": int(k + 1), "final_lagr": float(final),
+            "niter": niter, "tol": tol, "warm_start_s": None}
+
+
+if __name__ == "__main__":
+    main()

@@ -42,6 +42,14 @@ const char* pt_last_error(void);
 /* Library ABI version (increments on incompatible change). */
 int pt_abi_version(void);
 
+/* Launch profiler (no reference counterpart; measurement only).  After
+ * pt_prof_start(tag), every launch of the kernel named `tag` (e.g.
+ * "tomo_gather", "tomo_scatter", "ptycho_grad_tile") is bracketed by HIP
+ * events on its launch stream; pt_prof_stop synchronises on them and returns
+ * the summed kernel time and the launch count. */
+int pt_prof_start(const char* tag);
+int pt_prof_stop(double* total_ms, long* launches);
+
 /* ------------------------------------------------------------------ tomo
  * USFFT Radon transform on a slab of pnz slices of an n x n object.
  * f: [pnz, n, n] complex64, g: [ntheta, pnz, n] complex64.
@@ -171,20 +179,25 @@ int pt_k_takexi(void* xi0, void* K, const void* psi, const void* lamd, double rh
  * r = conj(K)*(K*Ru - xi0), out[0] = ||K*Ru - xi0||^2 */
 int pt_k_tomo_resid(void* r, double* out, const void* K, const void* Ru, const void* xi0, long len,
                     pt_stream stream);
-/* out[0] = Re sum conj(K*Ru - xi0)*(K*Rd), out[1] = ||K*Rd||^2 */
-int pt_k_tomo_ls(double* out, const void* K, const void* Ru, const void* Rd, const void* xi0,
-                 long len, pt_stream stream);
 /* grad += tau * J*(J u - xi1) (adj_reg of fwd_reg residual, solver.py:188-190),
  * out[0] = ||J u - xi1||^2.  u: [nz,ny,nx], xi1: [3,nz,ny,nx]. */
 int pt_k_reg_grad(void* grad, double* out, const void* u, const void* xi1, double tau, int nz,
                   int ny, int nx, pt_stream stream);
-/* out[0] = Re sum conj(J u - xi1)*J d, out[1] = ||J d||^2 */
-int pt_k_reg_ls(double* out, const void* u, const void* d, const void* xi1, int nz, int ny, int nx,
-                pt_stream stream);
-/* cg_ptycho rho term (solver.py:213) per segment, r0 = h - psi + lamd/rho:
- * out[3s] = ||r0||^2, out[3s+1] = Re sum conj(r0)*d, out[3s+2] = ||d||^2 */
-int pt_k_rho_terms(double* out, const void* h, const void* psi, const void* lamd, const void* d,
-                   double rho, long len, int nseg, pt_stream stream);
+/* Line-search candidates (solver.py:171-177), evaluated as the reference
+ * forms them: the stepped quantity x + gamma*dx element-wise in FP32, sums in
+ * FP64, so a step below FP32 resolution reproduces the gamma = 0 value
+ * exactly.  gammas: device FP64, ngam (<= 8) candidates per segment.
+ * cg_ptycho rho term (solver.py:213), per angle-chunk segment:
+ *   out[s*ngam+j] = ||h - (psi + g_j d) + lamd/rho||^2 */
+int pt_k_rho_minf(double* out, const void* h, const void* psi, const void* lamd, const void* d,
+                  double rho, const double* gammas, int ngam, long len, int nseg,
+                  pt_stream stream);
+/* cg_tomo (solver.py:182-183): out[j] = ||K*Ru + g_j*(K*Rd) - xi0||^2 */
+int pt_k_tomo_minf(double* out, const void* K, const void* Ru, const void* Rd, const void* xi0,
+                   const double* gammas, int ngam, long len, pt_stream stream);
+/* cg_tomo: out[j] = ||J u + g_j*J d - xi1||^2 */
+int pt_k_reg_minf(double* out, const void* u, const void* d, const void* xi1,
+                  const double* gammas, int ngam, int nz, int ny, int nx, pt_stream stream);
 /* out = sum conj(a)*(b - c) (complex, re/im) -- take_lagr terms 2 and 4 */
 int pt_k_dot_diff(double* out, const void* a, const void* b, const void* c, long len,
                   pt_stream stream);

@@ -28,6 +28,8 @@ PT_MODEL = {"gaussian": 0, "poisson": 1}
 _SIGS = {
     "pt_last_error": (ctypes.c_char_p, []),
     "pt_abi_version": (c_int, []),
+    "pt_prof_start": (c_int, [ctypes.c_char_p]),
+    "pt_prof_stop": (c_int, [ctypes.POINTER(c_double), ctypes.POINTER(c_long)]),
     "pt_tomo_create": (c_int, [c_int, c_int, c_int, ctypes.POINTER(vp)]),
     "pt_tomo_destroy": (c_int, [vp]),
     "pt_tomo_set_theta": (c_int, [vp, vp, vp]),
@@ -61,10 +63,10 @@ _SIGS = {
     "pt_k_takexi": (c_int, [vp, vp, vp, vp, c_double, c_double, c_double, c_double, c_double,
                             c_double, c_long, vp]),
     "pt_k_tomo_resid": (c_int, [vp, vp, vp, vp, vp, c_long, vp]),
-    "pt_k_tomo_ls": (c_int, [vp, vp, vp, vp, vp, c_long, vp]),
     "pt_k_reg_grad": (c_int, [vp, vp, vp, vp, c_double, c_int, c_int, c_int, vp]),
-    "pt_k_reg_ls": (c_int, [vp, vp, vp, vp, c_int, c_int, c_int, vp]),
-    "pt_k_rho_terms": (c_int, [vp, vp, vp, vp, vp, c_double, c_long, c_int, vp]),
+    "pt_k_rho_minf": (c_int, [vp, vp, vp, vp, vp, c_double, vp, c_int, c_long, c_int, vp]),
+    "pt_k_tomo_minf": (c_int, [vp, vp, vp, vp, vp, vp, c_int, c_long, vp]),
+    "pt_k_reg_minf": (c_int, [vp, vp, vp, vp, vp, c_int, c_int, c_int, c_int, vp]),
     "pt_k_dot_diff": (c_int, [vp, vp, vp, vp, c_long, vp]),
     "pt_k_tv_norm": (c_int, [vp, vp, c_long, vp]),
     "pt_k_lincomb3": (c_int, [vp, c_double, c_double, vp, c_double, c_double, vp, c_double,

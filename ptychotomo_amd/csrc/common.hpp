@@ -233,7 +233,7 @@ struct LinePlan {
   int radix[12];
 };
 
-#define PT_LDS_FFT_MAXV 24
+#define PT_LDS_FFT_MAXV 16
 
 // host: true if every pass of `p` fits the per-thread register budget
 inline bool lds_fft_fits(const LinePlan& p, int nlines, int nthr) {
@@ -246,93 +246,119 @@ inline bool lds_fft_fits(const LinePlan& p, int nlines, int nthr) {
   return true;
 }
 
-__device__ __forceinline__ void dft_rt(float2* a, int R, int sign) {
-  // generic small DFTs for run-time radix (sign = -1 forward, +1 inverse)
-  if (R == 2) {
-    dft2<1>(a[0], a[1]);
-  } else if (R == 4) {
-    if (sign < 0) dft4<-1>(a[0], a[1], a[2], a[3]);
-    else dft4<1>(a[0], a[1], a[2], a[3]);
-  } else if (R == 8) {
-    if (sign < 0) dft8<-1>(a);
-    else dft8<1>(a);
-  } else if (R == 3) {
-    const float s3 = 0.86602540378443864676f * (float)sign;
-    float2 t1 = cadd(a[1], a[2]);
-    float2 t2 = make_float2(a[0].x - 0.5f * t1.x, a[0].y - 0.5f * t1.y);
-    float2 d = csub(a[1], a[2]);
-    float2 t3 = make_float2(-s3 * d.y, s3 * d.x);  // sign*i*sqrt(3)/2*(a1-a2)
-    a[0] = cadd(a[0], t1);
-    a[1] = cadd(t2, t3);
-    a[2] = csub(t2, t3);
-  } else if (R == 5) {
-    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
-    const float s1 = 0.95105651629515357212f * (float)sign, s2 = 0.58778525229247312917f * (float)sign;
-    float2 b1 = cadd(a[1], a[4]), b4 = csub(a[1], a[4]);
-    float2 b2 = cadd(a[2], a[3]), b3 = csub(a[2], a[3]);
-    float2 x0 = a[0];
-    float2 r1 = make_float2(x0.x + c1 * b1.x + c2 * b2.x, x0.y + c1 * b1.y + c2 * b2.y);
-    float2 r2 = make_float2(x0.x + c2 * b1.x + c1 * b2.x, x0.y + c2 * b1.y + c1 * b2.y);
-    // i*(s1*b4 + s2*b3), i*(s2*b4 - s1*b3)
-    float2 q1 = make_float2(-(s1 * b4.y + s2 * b3.y), s1 * b4.x + s2 * b3.x);
-    float2 q2 = make_float2(-(s2 * b4.y - s1 * b3.y), s2 * b4.x - s1 * b3.x);
-    a[0] = make_float2(x0.x + b1.x + b2.x, x0.y + b1.y + b2.y);
-    a[1] = cadd(r1, q1);
-    a[4] = csub(r1, q1);
-    a[2] = cadd(r2, q2);
-    a[3] = csub(r2, q2);
+template <int SIGN>
+__device__ __forceinline__ void dft3(float2* a) {
+  constexpr float s3 = 0.86602540378443864676f * (float)SIGN;
+  const float2 t1 = cadd(a[1], a[2]);
+  const float2 t2 = make_float2(a[0].x - 0.5f * t1.x, a[0].y - 0.5f * t1.y);
+  const float2 d = csub(a[1], a[2]);
+  const float2 t3 = make_float2(-s3 * d.y, s3 * d.x);  // SIGN*i*sqrt(3)/2*(a1-a2)
+  a[0] = cadd(a[0], t1);
+  a[1] = cadd(t2, t3);
+  a[2] = csub(t2, t3);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void dft5(float2* a) {
+  constexpr float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+  constexpr float s1 = 0.95105651629515357212f * (float)SIGN;
+  constexpr float s2 = 0.58778525229247312917f * (float)SIGN;
+  const float2 b1 = cadd(a[1], a[4]), b4 = csub(a[1], a[4]);
+  const float2 b2 = cadd(a[2], a[3]), b3 = csub(a[2], a[3]);
+  const float2 x0 = a[0];
+  const float2 r1 = make_float2(x0.x + c1 * b1.x + c2 * b2.x, x0.y + c1 * b1.y + c2 * b2.y);
+  const float2 r2 = make_float2(x0.x + c2 * b1.x + c1 * b2.x, x0.y + c2 * b1.y + c1 * b2.y);
+  const float2 q1 = make_float2(-(s1 * b4.y + s2 * b3.y), s1 * b4.x + s2 * b3.x);
+  const float2 q2 = make_float2(-(s2 * b4.y - s1 * b3.y), s2 * b4.x - s1 * b3.x);
+  a[0] = make_float2(x0.x + b1.x + b2.x, x0.y + b1.y + b2.y);
+  a[1] = cadd(r1, q1);
+  a[4] = csub(r1, q1);
+  a[2] = cadd(r2, q2);
+  a[3] = csub(r2, q2);
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_small(float2* a) {
+  if constexpr (R == 3) dft3<SIGN>(a);
+  else if constexpr (R == 5) dft5<SIGN>(a);
+  else dft_ct<R, SIGN>(a);
+}
+
+// One in-place Stockham pass of radix R over `nlines` lines; each thread
+// keeps at most PT_LDS_FFT_MAXV values (compile-time bound, so they stay in
+// VGPRs).  Twiddle W_{Ns R}^{i k} = tw[i k N/(Ns R)] (conjugated for SIGN=+1).
+template <int R, int SIGN>
+__device__ __forceinline__ void lds_pass(float2* buf, int stride, int nlines, int N, int Ns,
+                                         const float2* __restrict__ tw, int t, int nthr) {
+  constexpr int MAXB = PT_LDS_FFT_MAXV / R;
+  const int nb = N / R;
+  const int total = nb * nlines;
+  const int step = N / (Ns * R);
+  float2 v[MAXB][R];
+#pragma unroll
+  for (int bi = 0; bi < MAXB; ++bi) {
+    const int b = t + bi * nthr;
+    if (b < total) {
+      const int line = b / nb, j = b - line * nb;
+      const int k = j % Ns;
+      const float2* src = buf + line * stride;
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        float2 x = src[j + i * nb];
+        if (i) {
+          const int m = i * k * step;
+          if (m) {
+            float2 w = tw[m];
+            if (SIGN > 0) w.y = -w.y;
+            x = cmul(x, w);
+          }
+        }
+        v[bi][i] = x;
+      }
+      dft_small<R, SIGN>(v[bi]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int bi = 0; bi < MAXB; ++bi) {
+    const int b = t + bi * nthr;
+    if (b < total) {
+      const int line = b / nb, j = b - line * nb;
+      const int k = j % Ns;
+      float2* dst = buf + line * stride;
+      const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int i = 0; i < R; ++i) dst[idxD + i * Ns] = v[bi][i];
+    }
+  }
+  __syncthreads();
+}
+
+template <int SIGN>
+__device__ inline void lds_fft_lines_t(float2* buf, int stride, int nlines, const LinePlan& plan,
+                                       const float2* __restrict__ tw, int t, int nthr) {
+  int Ns = 1;
+  for (int p = 0; p < plan.npass; ++p) {
+    const int R = plan.radix[p];
+    switch (R) {
+      case 2: lds_pass<2, SIGN>(buf, stride, nlines, plan.N, Ns, tw, t, nthr); break;
+      case 3: lds_pass<3, SIGN>(buf, stride, nlines, plan.N, Ns, tw, t, nthr); break;
+      case 4: lds_pass<4, SIGN>(buf, stride, nlines, plan.N, Ns, tw, t, nthr); break;
+      case 5: lds_pass<5, SIGN>(buf, stride, nlines, plan.N, Ns, tw, t, nthr); break;
+      default: lds_pass<8, SIGN>(buf, stride, nlines, plan.N, Ns, tw, t, nthr); break;
+    }
+    Ns *= R;
   }
 }
 
 // In-place FFT of `nlines` lines of length plan.N stored in LDS at
-// buf + line*stride; executed by `nthr` threads (thread index t) that all call
+// buf + line*stride, executed by `nthr` threads (thread index t) that all call
 // it together; contains __syncthreads().  sign = -1 forward, +1 inverse
-// (unnormalised).  The caller guarantees ceil(nlines*N/R/nthr)*R <= PT_LDS_FFT_MAXV
-// for every pass radix R (checked on the host by lds_fft_fits()).
+// (unnormalised).  The caller guarantees lds_fft_fits(plan, nlines, nthr).
 __device__ inline void lds_fft_lines(float2* buf, int stride, int nlines, const LinePlan& plan,
                                      const float2* __restrict__ tw, int sign, int t, int nthr) {
-  const int N = plan.N;
-  int Ns = 1;
-  for (int p = 0; p < plan.npass; ++p) {
-    const int R = plan.radix[p];
-    const int nb = N / R;               // butterflies per line
-    const int total = nb * nlines;
-    float2 v[PT_LDS_FFT_MAXV];
-    int cnt = 0;
-    // load + twiddle + butterfly
-    for (int b = t; b < total && cnt + R <= PT_LDS_FFT_MAXV; b += nthr) {
-      int line = b / nb, j = b - line * nb;
-      int k = j % Ns;
-      float2* src = buf + line * stride;
-      float2 a[8];
-      const int step = N / (Ns * R);
-      for (int i = 0; i < R; ++i) {
-        float2 x = src[j + i * nb];
-        if (i && k) {
-          int m = (i * k * step) % N;
-          float2 w = tw[m];
-          if (sign > 0) w.y = -w.y;
-          x = cmul(x, w);
-        }
-        a[i] = x;
-      }
-      dft_rt(a, R, sign);
-      for (int i = 0; i < R; ++i) v[cnt + i] = a[i];
-      cnt += R;
-    }
-    __syncthreads();
-    cnt = 0;
-    for (int b = t; b < total && cnt + R <= PT_LDS_FFT_MAXV; b += nthr) {
-      int line = b / nb, j = b - line * nb;
-      int k = j % Ns;
-      float2* dst = buf + line * stride;
-      int idxD = (j / Ns) * Ns * R + k;
-      for (int i = 0; i < R; ++i) dst[idxD + i * Ns] = v[cnt + i];
-      cnt += R;
-    }
-    __syncthreads();
-    Ns *= R;
-  }
+  if (sign < 0) lds_fft_lines_t<-1>(buf, stride, nlines, plan, tw, t, nthr);
+  else lds_fft_lines_t<1>(buf, stride, nlines, plan, tw, t, nthr);
 }
 
 // ---------------------------------------------------------------- reductions
@@ -368,5 +394,15 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* smem /* >= NV
 
 // host: factor N into the supported radices; returns false if impossible
 bool make_line_plan(int N, LinePlan* p);
+
+// Launch profiler: when enabled for a kernel tag, HIP events are recorded on
+// the launch stream right before and after every launch of that kernel.
+struct ProfScope {
+  bool active;
+  int slot;
+  hipStream_t st;
+  ProfScope(const char* tag, hipStream_t stream);
+  ~ProfScope();
+};
 
 }  // namespace pt

@@ -183,14 +183,14 @@ __global__ __launch_bounds__(WAVE) void k_minf(double* __restrict__ partial,
       for (int k = 0; k < D; ++k) {
         const float d = dp[k * D] * cd;
         const float2 A = cscale(a[k], cp), Bv = cscale(b[k], cp);
-        const float aa = A.x * A.x + A.y * A.y;
-        const float ab = A.x * Bv.x + A.y * Bv.y;
-        const float bb = Bv.x * Bv.x + Bv.y * Bv.y;
         float dterm = 0.f;
         if (lagr && model == PT_MODEL_POISSON) dterm = d - 2.f * d * logf(fmaxf(sqrtf(d), 1e-32f));
         for (int j = 0; j < ngam; ++j) {
+          // fpsi + gamma*fd formed element-wise in FP32 (solver.py:172): a step
+          // below FP32 resolution leaves the term bit-identical to gamma = 0
           const float gj = gam[j];
-          const float s = fmaxf(aa + gj * (2.f * ab + gj * bb), 0.f);
+          const float2 ag = make_float2(A.x + gj * Bv.x, A.y + gj * Bv.y);
+          const float s = ag.x * ag.x + ag.y * ag.y;
           const float m = sqrtf(s);
           float term;
           if (model == PT_MODEL_POISSON) {
@@ -386,6 +386,7 @@ int run_fwd(pt_ptycho* h, float2* g, const float2* f, float scale, hipStream_t s
   constexpr int G = WAVE / D;
   long np = (long)h->pt * h->S;
   int nb = launch_blocks(np, G, 1 << 20);
+  ProfScope prof("ptycho_fwd", st);
   hipLaunchKernelGGL(k_fwd<D>, dim3(nb), dim3(WAVE), 0, st, g, f, h->prb, h->pos, h->shift, h->pt,
                      h->nz, h->n, h->S, h->P, h->ox, h->oy, h->c * scale);
   PT_LAUNCH_CHECK();
@@ -397,6 +398,7 @@ int run_tiles(pt_ptycho* h, const float2* gin, const float2* f, const float* dat
               float cd, float out_scale, int model, hipStream_t st) {
   constexpr int G = WAVE / D;
   size_t lds = sizeof(float2) * ((size_t)G * D * (D + 1) + (size_t)h->EB * h->EB);
+  ProfScope prof(MODE == TILE_GRAD ? "ptycho_grad_tile" : "ptycho_adj_tile", st);
   hipLaunchKernelGGL((k_tile<D, MODE>), dim3((unsigned)h->ntiles), dim3(WAVE), lds, st, h->scratch,
                      gin, f, data, h->prb, h->pos, h->shift, h->tile_start, h->tile_patch,
                      h->tile_slot, h->nty, h->ntx, h->B, h->EB, h->nz, h->n, h->S, h->P, h->ox,
@@ -420,9 +422,12 @@ int run_minf(pt_ptycho* h, double* sums, const float2* f, const float2* dir, con
     PT_HIP(hipMalloc(&h->partial, need * sizeof(double)));
     h->partial_cap = need;
   }
+  {
+  ProfScope prof("ptycho_minf", st);
   hipLaunchKernelGGL(k_minf<D>, dim3(nb, nchunk), dim3(WAVE), 0, st, h->partial, f, dir, data,
                      h->prb, h->pos, gammas, ngam, chunk, h->nz, h->n, h->S, h->P, h->ox, h->oy,
                      h->c, cp, cd, model, lagr);
+  }
   PT_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_reduce_partials, dim3(nchunk), dim3(64), 0, st, sums, h->partial, nb, ngam);
   PT_LAUNCH_CHECK();
@@ -434,6 +439,7 @@ int run_fold(pt_ptycho* h, float2* out, const float2* psi, const float2* hh, con
              float rho, hipStream_t st) {
   long total = (long)h->pt * h->nz * h->n;
   int nb = (int)((total + 255) / 256);
+  ProfScope prof("ptycho_fold", st);
   hipLaunchKernelGGL(k_fold<MODE>, dim3(nb), dim3(256), 0, st, out, h->scratch, h->tile_slot, psi,
                      hh, lamd, rho, h->pt, h->nz, h->n, h->nty, h->ntx, h->B, h->EB);
   PT_LAUNCH_CHECK();

@@ -51,16 +51,24 @@ struct pt_tomo {
   int2* sfloor = nullptr;    // [ntheta*N] floor(2N*x), floor(2N*y)
   float2* A = nullptr;       // [pz][N][2N]
   float2* Bg = nullptr;      // [pz][2N][2N]
-  float2* Q = nullptr;       // [pz][ntheta*N] polar samples (adjoint)
-  // scatter bins
+  float2* Q = nullptr;       // polar samples: fwd G[z][sample], adj Q[sample][z]
+  // gather bins: samples by kTile x kTile tile of their floor position
   int ntile1 = 0;            // tiles per axis
   int* wi_tile = nullptr;    // [nwi] tile index
   int2* wi_range = nullptr;  // [nwi] [s0, s1) into bin_sample
   int* bin_sample = nullptr; // [ntheta*N]
-  int* tile_wi = nullptr;    // [ntile1^2 + 1] CSR: work items of each tile
   int nwi = 0;
-  float2* scratch = nullptr; // [nwi][pz][(T+2M)^2]
-  int rows_per_wg = 1, cols_per_wg = 1;
+  // scatter: per kTS x kTS grid tile, the samples whose footprint covers it
+  int nt8 = 0;
+  int* ent_s = nullptr;      // [nent] sample
+  int* ent_o = nullptr;      // [nent] packed footprint offset (ox+64) | (oy+64) << 8
+  int* swi_tile = nullptr;   // [nswi]
+  int2* swi_range = nullptr; // [nswi] entry range
+  int* swi_slot = nullptr;   // [nswi] partial slot, -1 = direct write
+  int4* sfix = nullptr;      // [nfix] (tile, first slot, nslots, 0)
+  int nswi = 0, nfix = 0;
+  float2* spart = nullptr;   // [nslots][pz][64]
+  int rows_per_wg = 1, cols_per_wg = 1, adj_lines = 1;
   bool ready = false;
 };
 
@@ -76,7 +84,7 @@ __device__ __forceinline__ int wrap2n(int v, int twoN) {
 // ------------------------------------------------------------------ forward row pass
 // For rows q of slice z: x[p] = mask*u[clamp]*dphi[q]*dphi[p] (pad2, circ,
 // divphi), centred 2N-point DFT along p -> A[z][q][kx], kx = k' + N.
-__global__ void k_row_fwd(float2* __restrict__ A, const float2* __restrict__ u,
+__global__ __launch_bounds__(256) void k_row_fwd(float2* __restrict__ A, const float2* __restrict__ u,
                           const float2* __restrict__ tw, const float2* __restrict__ twist,
                           const float* __restrict__ dphi, const int2* __restrict__ mrange,
                           LinePlan plan, int n, int rows) {
@@ -102,7 +110,7 @@ __global__ void k_row_fwd(float2* __restrict__ A, const float2* __restrict__ u,
     lds[line * N + a] = val;
   }
   __syncthreads();
-  lds_fft_lines(lds, N, nl, plan, tw, -1, threadIdx.x, blockDim.x);
+  lds_fft_lines_t<-1>(lds, N, nl, plan, tw, threadIdx.x, blockDim.x);
   for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
     const int line = i / N, b = i - line * N;
     const int r = line >> 1, q = q0 + r;
@@ -116,7 +124,7 @@ __global__ void k_row_fwd(float2* __restrict__ A, const float2* __restrict__ u,
 // ------------------------------------------------------------------ forward column pass
 // For columns kx of slice z: y[q] = A[z][q][kx] -> centred 2N-point DFT along
 // q -> B[z][ky][kx].
-__global__ void k_col_fwd(float2* __restrict__ Bg, const float2* __restrict__ A,
+__global__ __launch_bounds__(256) void k_col_fwd(float2* __restrict__ Bg, const float2* __restrict__ A,
                           const float2* __restrict__ tw, const float2* __restrict__ twist,
                           LinePlan plan, int cols) {
   extern __shared__ float2 lds[];
@@ -131,7 +139,7 @@ __global__ void k_col_fwd(float2* __restrict__ Bg, const float2* __restrict__ A,
     lds[(2 * cc + 1) * N + a] = cmul(val, twist[q]);
   }
   __syncthreads();
-  lds_fft_lines(lds, N, nl, plan, tw, -1, threadIdx.x, blockDim.x);
+  lds_fft_lines_t<-1>(lds, N, nl, plan, tw, threadIdx.x, blockDim.x);
   for (int i = threadIdx.x; i < 2 * N * cols; i += blockDim.x) {
     const int ky = i / cols, cc = i - ky * cols;
     const int kp = ky - N;                    // centred k' in [-N, N)
@@ -142,18 +150,41 @@ __global__ void k_col_fwd(float2* __restrict__ Bg, const float2* __restrict__ A,
   }
 }
 
-// ------------------------------------------------------------------ gather + 1D IDFT
-__global__ void k_gather_line(float2* __restrict__ g, const float2* __restrict__ Bg,
-                              const float2* __restrict__ spos, const int2* __restrict__ sfloor,
-                              const float2* __restrict__ tw, LinePlan plan, int M, float beta,
-                              float oscale, int n, int ntheta, int pz) {
-  extern __shared__ float2 lds[];
-  const int N = plan.N, twoN = 2 * N;
-  const int th = blockIdx.x, z = blockIdx.y;
+// ------------------------------------------------------------------ gather
+// One workgroup per (32 x 32 grid tile bin chunk, slice pair): the tile plus
+// an M-point halo (periodic in 2N) is staged in LDS as [y][x][z-pair]; each
+// lane interpolates one polar sample of the bin with separable Gaussian
+// weights (kernels.cuh:97-126).  Output G[z][sample].
+__global__ __launch_bounds__(256) void k_gather_tile(float2* __restrict__ G,
+                                                     const float2* __restrict__ Bg,
+                                                     const float2* __restrict__ spos,
+                                                     const int2* __restrict__ sfloor,
+                                                     const int* __restrict__ wi_tile,
+                                                     const int2* __restrict__ wi_range,
+                                                     const int* __restrict__ bin_sample, int N,
+                                                     int M, float beta, int ntile1, int ns,
+                                                     int pz) {
+  extern __shared__ float4 lds4[];  // [E][E] pairs of complex (z0, z0+1)
+  const int twoN = 2 * N, E = kTile + 2 * M;
+  const int wi = blockIdx.x, z0 = 2 * blockIdx.y;
+  const bool two = z0 + 1 < pz;
+  const int tile = wi_tile[wi];
+  const int ty0 = (tile / ntile1) * kTile - N, tx0 = (tile % ntile1) * kTile - N;
+  const float2* B0 = Bg + (long)z0 * twoN * twoN;
+  const float2* B1 = two ? B0 + (long)twoN * twoN : B0;
+  for (int i = threadIdx.x; i < E * E; i += blockDim.x) {
+    const int y = i / E, x = i - y * E;
+    const int gy = wrap2n(N + ty0 - M + y, twoN), gx = wrap2n(N + tx0 - M + x, twoN);
+    const long off = (long)gy * twoN + gx;
+    const float2 a = B0[off];
+    const float2 b = two ? B1[off] : make_float2(0.f, 0.f);
+    lds4[i] = make_float4(a.x, a.y, b.x, b.y);
+  }
+  __syncthreads();
+  const int2 rg = wi_range[wi];
   const float inv2N = 1.0f / (float)twoN;
-  const float2* Bz = Bg + (long)z * twoN * twoN;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) {
-    const long s = (long)th * N + i;
+  for (int bi = rg.x + threadIdx.x; bi < rg.y; bi += blockDim.x) {
+    const int s = bin_sample[bi];
     const float2 p = spos[s];
     const int2 fl = sfloor[s];
     float ex[15], ey[15];
@@ -164,32 +195,46 @@ __global__ void k_gather_line(float2* __restrict__ g, const float2* __restrict__
       ex[k] = (k <= 2 * M) ? __expf(-beta * dx * dx) : 0.f;
       ey[k] = (k <= 2 * M) ? __expf(-beta * dy * dy) : 0.f;
     }
-    float2 acc = make_float2(0.f, 0.f);
-    int cx[15];
-#pragma unroll
-    for (int k = 0; k < 15; ++k) cx[k] = wrap2n(N + fl.x - M + k, twoN);
+    const int row = fl.y - ty0, col = fl.x - tx0;  // footprint origin in the staged tile
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int k1 = 0; k1 <= 2 * M; ++k1) {
-      const float2* row = Bz + (long)wrap2n(N + fl.y - M + k1, twoN) * twoN;
-      float2 r = make_float2(0.f, 0.f);
+      const float4* rp = lds4 + (row + k1) * E + col;
+      float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
       for (int k0 = 0; k0 < 15; ++k0) {
         if (k0 <= 2 * M) {
-          const float2 v = row[cx[k0]];
+          const float4 v = rp[k0];
           r.x += ex[k0] * v.x;
           r.y += ex[k0] * v.y;
+          r.z += ex[k0] * v.z;
+          r.w += ex[k0] * v.w;
         }
       }
       acc.x += ey[k1] * r.x;
       acc.y += ey[k1] * r.y;
+      acc.z += ey[k1] * r.z;
+      acc.w += ey[k1] * r.w;
     }
-    // kappa = i - N/2 -> rotated index kappa mod N
-    int a = i - N / 2;
+    G[(long)z0 * ns + s] = make_float2(acc.x, acc.y);
+    if (two) G[(long)(z0 + 1) * ns + s] = make_float2(acc.z, acc.w);
+  }
+}
+
+// centred inverse 1D DFT of each (angle, slice) line of gathered samples,
+// fftshift1c signs folded into index rotations, detector crop
+__global__ __launch_bounds__(256) void k_line_idft(float2* __restrict__ g, const float2* __restrict__ G,
+                            const float2* __restrict__ tw, LinePlan plan, float oscale, int n,
+                            int ntheta, int pz) {
+  extern __shared__ float2 lds[];
+  const int N = plan.N, th = blockIdx.x, z = blockIdx.y;
+  const float2* Gl = G + (long)z * ntheta * N + (long)th * N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    int a = i - N / 2;  // kappa -> kappa mod N
     if (a < 0) a += N;
-    lds[a] = acc;
+    lds[a] = Gl[i];
   }
   __syncthreads();
-  lds_fft_lines(lds, N, 1, plan, tw, +1, threadIdx.x, blockDim.x);
-  // crop: output j in [0, n) <-> sigma = j - n/2
+  lds_fft_lines_t<1>(lds, N, 1, plan, tw, threadIdx.x, blockDim.x);
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     int b = j - n / 2;
     if (b < 0) b += N;
@@ -198,121 +243,136 @@ __global__ void k_gather_line(float2* __restrict__ g, const float2* __restrict__
 }
 
 // ------------------------------------------------------------------ adjoint line: pad + 1D DFT
-__global__ void k_adj_line(float2* __restrict__ Q, const float2* __restrict__ g,
+// ZQ slices of one angle per workgroup; output Q[sample][z] (z fastest) for
+// the owner-computes scatter.
+constexpr int kZQ = 8;
+__global__ __launch_bounds__(256) void k_adj_line(float2* __restrict__ Q, const float2* __restrict__ g,
                            const float2* __restrict__ tw, LinePlan plan, float oscale, int n,
-                           int ntheta, int pz) {
+                           int ntheta, int pz, int zq) {
   extern __shared__ float2 lds[];
-  const int N = plan.N, th = blockIdx.x, z = blockIdx.y;
+  const int N = plan.N, th = blockIdx.x, z0 = blockIdx.y * zq;
+  const int nzl = min(zq, pz - z0);
   const int o = N / 6;
-  const float2* gl = g + ((long)th * pz + z) * n;
-  for (int s = threadIdx.x; s < N; s += blockDim.x) {
-    const int j = min(max(s - o, 0), n - 1);  // pad (kernels.cuh:189-206)
-    int a = s + N / 2;
+  for (int i = threadIdx.x; i < zq * N; i += blockDim.x) {
+    const int zz = i / N, sidx = i - zz * N;
+    float2 v = make_float2(0.f, 0.f);
+    if (zz < nzl) {
+      const int j = min(max(sidx - o, 0), n - 1);  // pad (kernels.cuh:189-206)
+      v = g[((long)th * pz + z0 + zz) * n + j];
+    }
+    int a = sidx + N / 2;
     if (a >= N) a -= N;
-    lds[a] = gl[j];
+    lds[zz * N + a] = v;
   }
   __syncthreads();
-  lds_fft_lines(lds, N, 1, plan, tw, -1, threadIdx.x, blockDim.x);
-  float2* Ql = Q + ((long)z * ntheta + th) * N;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+  lds_fft_lines_t<-1>(lds, N, zq, plan, tw, threadIdx.x, blockDim.x);
+  for (int t = threadIdx.x; t < zq * N; t += blockDim.x) {
+    const int i = t / zq, zz = t - i * zq;
+    if (zz >= nzl) continue;
     int b = i - N / 2;  // kappa
     if (b < 0) b += N;
-    Ql[i] = cscale(lds[b], oscale);
+    Q[((long)th * N + i) * pz + z0 + zz] = cscale(lds[zz * N + b], oscale);
   }
 }
 
-// ------------------------------------------------------------------ scatter
-// One wavefront per (work item, slice).  Lanes: slot = lane/16 (4 samples in
-// flight), c = lane%16 (footprint column, c <= 2M).
-__global__ __launch_bounds__(64) void k_scatter(float2* __restrict__ scratch,
-                                                const float2* __restrict__ Q,
-                                                const float2* __restrict__ spos,
-                                                const int2* __restrict__ sfloor,
-                                                const int* __restrict__ wi_tile,
-                                                const int2* __restrict__ wi_range,
-                                                const int* __restrict__ bin_sample, int N, int M,
-                                                float beta, int ntile1, int ns_total, int pz) {
-  extern __shared__ float2 acc[];
-  const int wi = blockIdx.x, z = blockIdx.y;
-  const int E = kTile + 2 * M;
-  const int lane = threadIdx.x, slot = lane >> 4, c = lane & 15;
-  for (int i = lane; i < E * E; i += 64) acc[i] = make_float2(0.f, 0.f);
-  __syncthreads();
-  const int tile = wi_tile[wi];
-  const int ty0 = (tile / ntile1) * kTile - N, tx0 = (tile % ntile1) * kTile - N;
-  const int2 rg = wi_range[wi];
-  const float inv2N = 1.0f / (float)(2 * N);
-  const float2* Qz = Q + (long)z * ns_total;
-  for (int b = rg.x; b < rg.y; b += 4) {
-    const int bi = b + slot;
-    const bool active = bi < rg.y && c <= 2 * M;
-    const int s = bi < rg.y ? bin_sample[bi] : 0;
-    const float2 p = spos[s];
-    const int2 fl = sfloor[s];
-    const float2 qv = Qz[s];
-    const float dx = (float)(fl.x - M + c) * inv2N - p.x;
-    const float dy = (float)(fl.y - M + c) * inv2N - p.y;
-    const float ex = __expf(-beta * dx * dx);
-    const float eyc = __expf(-beta * dy * dy);
-    const float2 qx = make_float2(qv.x * ex, qv.y * ex);
-    const int col = fl.x - tx0 + c;  // in [0, E)
-    const int rowb = fl.y - ty0;     // row of k1 = 0 is rowb (+M offset cancels)
-    for (int k1 = 0; k1 <= 2 * M; ++k1) {
-      const float ey = __shfl(eyc, (slot << 4) + k1, 64);
-      if (active) {
-        float2* dst = acc + (rowb + k1) * E + col;
-        atomicAdd(&dst->x, qx.x * ey);
-        atomicAdd(&dst->y, qx.y * ey);
-      }
-    }
-  }
-  __syncthreads();
-  float2* out = scratch + ((long)wi * pz + z) * E * E;
-  for (int i = lane; i < E * E; i += 64) out[i] = acc[i];
-}
-
-// fold the extended tiles into the periodic 2N x 2N grid (fixed order)
-__global__ void k_fold_grid(float2* __restrict__ Bg, const float2* __restrict__ scratch,
-                            const int* __restrict__ tile_wi, int N, int M, int ntile1, int pz) {
+// ------------------------------------------------------------------ scatter (owner computes)
+// One wavefront per (8 x 8 grid tile entry chunk, kZS slices).  Lane = grid
+// point; it sums, in the fixed order of the tile's entry list, the weighted
+// samples whose 15 x 15 footprint covers it (kernels.cuh:128-158 transposed,
+// wrapadj folded in by the periodic entry offsets).  Weights and sample
+// values of 64 entries at a time are staged in LDS.
+constexpr int kTS = 8;    // scatter tile edge
+constexpr int kZS = 16;   // slices per work item
+constexpr int kSChunk = 1024;
+__global__ __launch_bounds__(64) void k_scatter_own(float2* __restrict__ Bg,
+                                                    float2* __restrict__ part,
+                                                    const float2* __restrict__ Q,
+                                                    const float2* __restrict__ spos,
+                                                    const int2* __restrict__ sfloor,
+                                                    const int* __restrict__ ent_s,
+                                                    const int* __restrict__ ent_o,
+                                                    const int* __restrict__ wi_tile,
+                                                    const int2* __restrict__ wi_range,
+                                                    const int* __restrict__ wi_slot, int N,
+                                                    int M, float beta, int nt8, int pz) {
+  __shared__ float wxs[64][16];
+  __shared__ float wys[64][16];
+  __shared__ float2 gs[64][kZS];
+  __shared__ int eos[64];
   const int twoN = 2 * N;
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long total = (long)pz * twoN * twoN;
-  if (idx >= total) return;
-  const int kx = (int)(idx % twoN);
-  const long r = idx / twoN;
-  const int ky = (int)(r % twoN);
-  const int z = (int)(r / twoN);
-  const int E = kTile + 2 * M;
-  float2 s = make_float2(0.f, 0.f);
-  for (int iy = -1; iy <= 1; ++iy) {
-    const int ey = ky - N + iy * twoN;  // image of ell_y
-    // tiles ty with ty*T - N - M <= ey < ty*T - N + T + M
-    const int tlo = (int)floorf((float)(ey + N + M - E + 1) / kTile);
-    const int thi = (int)floorf((float)(ey + N + M) / kTile);
-    for (int ty = max(tlo, 0); ty <= min(thi, ntile1 - 1); ++ty) {
-      const int ly = ey - (ty * kTile - N - M);
-      if (ly < 0 || ly >= E) continue;
-      for (int ix = -1; ix <= 1; ++ix) {
-        const int ex = kx - N + ix * twoN;
-        const int slo = (int)floorf((float)(ex + N + M - E + 1) / kTile);
-        const int shi = (int)floorf((float)(ex + N + M) / kTile);
-        for (int tx = max(slo, 0); tx <= min(shi, ntile1 - 1); ++tx) {
-          const int lx = ex - (tx * kTile - N - M);
-          if (lx < 0 || lx >= E) continue;
-          const int tile = ty * ntile1 + tx;
-          for (int w = tile_wi[tile]; w < tile_wi[tile + 1]; ++w)
-            s = cadd(s, scratch[((long)w * pz + z) * E * E + ly * E + lx]);
+  const int wi = blockIdx.x, z0 = blockIdx.y * kZS;
+  const int lane = threadIdx.x, lx = lane & 7, ly = lane >> 3;
+  const int tile = wi_tile[wi];
+  const int gx = (tile % nt8) * kTS + lx, gy = (tile / nt8) * kTS + ly;
+  const int2 rg = wi_range[wi];
+  const float inv2N = 1.0f / (float)twoN;
+  float2 acc[kZS];
+#pragma unroll
+  for (int j = 0; j < kZS; ++j) acc[j] = make_float2(0.f, 0.f);
+  for (int c0 = rg.x; c0 < rg.y; c0 += 64) {
+    const int ne = min(64, rg.y - c0);
+    __syncthreads();
+    if (lane < ne) {
+      const int s = ent_s[c0 + lane];
+      const float2 p = spos[s];
+      const int2 fl = sfloor[s];
+#pragma unroll
+      for (int k = 0; k < 15; ++k) {
+        const float dx = (float)(fl.x - M + k) * inv2N - p.x;
+        const float dy = (float)(fl.y - M + k) * inv2N - p.y;
+        wxs[lane][k] = (k <= 2 * M) ? __expf(-beta * dx * dx) : 0.f;
+        wys[lane][k] = (k <= 2 * M) ? __expf(-beta * dy * dy) : 0.f;
+      }
+      eos[lane] = ent_o[c0 + lane];
+      const float2* qs = Q + (long)s * pz + z0;
+#pragma unroll
+      for (int j = 0; j < kZS; ++j) gs[lane][j] = (z0 + j < pz) ? qs[j] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    for (int e = 0; e < ne; ++e) {
+      const int o = eos[e];
+      const int k0 = lx - ((o & 255) - 64);
+      const int k1 = ly - (((o >> 8) & 255) - 64);
+      if ((unsigned)k0 <= (unsigned)(2 * M) && (unsigned)k1 <= (unsigned)(2 * M)) {
+        const float w = wxs[e][k0] * wys[e][k1];
+#pragma unroll
+        for (int j = 0; j < kZS; ++j) {
+          const float2 q = gs[e][j];
+          acc[j].x += w * q.x;
+          acc[j].y += w * q.y;
         }
       }
     }
   }
-  Bg[idx] = s;
+  if (gx >= twoN || gy >= twoN) return;
+  const int slot = wi_slot[wi];
+#pragma unroll
+  for (int j = 0; j < kZS; ++j) {
+    if (z0 + j >= pz) break;
+    if (slot < 0) Bg[((long)(z0 + j) * twoN + gy) * twoN + gx] = acc[j];
+    else part[((long)slot * pz + z0 + j) * 64 + lane] = acc[j];
+  }
+}
+
+// tiles whose entry list spans several chunks: sum the chunk partials in order
+__global__ __launch_bounds__(64) void k_scatter_fix(float2* __restrict__ Bg,
+                                                    const float2* __restrict__ part,
+                                                    const int4* __restrict__ fix, int N, int nt8,
+                                                    int pz) {
+  const int4 f = fix[blockIdx.x];  // (tile, first slot, nslots, -)
+  const int z = blockIdx.y, lane = threadIdx.x;
+  const int twoN = 2 * N;
+  const int gx = (f.x % nt8) * kTS + (lane & 7), gy = (f.x / nt8) * kTS + (lane >> 3);
+  if (gx >= twoN || gy >= twoN) return;
+  float2 s = make_float2(0.f, 0.f);
+  for (int k = 0; k < f.z; ++k) s = cadd(s, part[((long)(f.y + k) * pz + z) * 64 + lane]);
+  Bg[((long)z * twoN + gy) * twoN + gx] = s;
 }
 
 // ------------------------------------------------------------------ adjoint column pass
 // For columns kx: centred inverse 2N-point DFT along ky evaluated at the n
 // object rows q in [o, o+n): even/odd split into two N-point inverse DFTs.
-__global__ void k_col_adj(float2* __restrict__ A, const float2* __restrict__ Bg,
+__global__ __launch_bounds__(256) void k_col_adj(float2* __restrict__ A, const float2* __restrict__ Bg,
                           const float2* __restrict__ tw, const float2* __restrict__ twist,
                           LinePlan plan, int n, int cols) {
   extern __shared__ float2 lds[];
@@ -327,7 +387,7 @@ __global__ void k_col_adj(float2* __restrict__ A, const float2* __restrict__ Bg,
     lds[(2 * cc + par) * N + b] = Bg[((long)z * 2 * N + ky) * (2 * N) + k0 + cc];
   }
   __syncthreads();
-  lds_fft_lines(lds, N, nl, plan, tw, +1, threadIdx.x, blockDim.x);
+  lds_fft_lines_t<1>(lds, N, nl, plan, tw, threadIdx.x, blockDim.x);
   const int o = N / 6;
   for (int i = threadIdx.x; i < n * cols; i += blockDim.x) {
     const int qi = i / cols, cc = i - qi * cols;
@@ -341,7 +401,7 @@ __global__ void k_col_adj(float2* __restrict__ A, const float2* __restrict__ Bg,
 }
 
 // ------------------------------------------------------------------ adjoint row pass
-__global__ void k_row_adj(float2* __restrict__ f, const float2* __restrict__ A,
+__global__ __launch_bounds__(256) void k_row_adj(float2* __restrict__ f, const float2* __restrict__ A,
                           const float2* __restrict__ tw, const float2* __restrict__ twist,
                           const float* __restrict__ dphi, const int2* __restrict__ mrange,
                           LinePlan plan, int n, int rows, float scale) {
@@ -361,7 +421,7 @@ __global__ void k_row_adj(float2* __restrict__ f, const float2* __restrict__ A,
     lds[(2 * r + par) * N + b] = val;
   }
   __syncthreads();
-  lds_fft_lines(lds, N, nl, plan, tw, +1, threadIdx.x, blockDim.x);
+  lds_fft_lines_t<1>(lds, N, nl, plan, tw, threadIdx.x, blockDim.x);
   for (int i = threadIdx.x; i < n * rows; i += blockDim.x) {
     const int r = i / n, pi = i - r * n;
     const int qi = r0 + r;
@@ -396,6 +456,7 @@ int run_fwd(pt_tomo* h, float2* g, const float2* f, float scale, hipStream_t st)
     const int rows = h->rows_per_wg;
     dim3 grid((N + rows - 1) / rows, h->pz);
     size_t lds = sizeof(float2) * 2 * rows * N;
+    ProfScope prof("tomo_row_fwd", st);
     hipLaunchKernelGGL(k_row_fwd, grid, dim3(nthr), lds, st, h->A, f, h->tw, h->twist, h->dphi,
                        h->mrange, h->plan, h->n, rows);
     PT_LAUNCH_CHECK();
@@ -404,19 +465,30 @@ int run_fwd(pt_tomo* h, float2* g, const float2* f, float scale, hipStream_t st)
     const int cols = h->cols_per_wg;
     dim3 grid(2 * N / cols, h->pz);
     size_t lds = sizeof(float2) * 2 * cols * N;
+    ProfScope prof("tomo_col_fwd", st);
     hipLaunchKernelGGL(k_col_fwd, grid, dim3(nthr), lds, st, h->Bg, h->A, h->tw, h->twist, h->plan,
                        cols);
     PT_LAUNCH_CHECK();
   }
   {
     const float beta = (float)(kPiRef * kPiRef / (double)h->mu);
+    const int E = kTile + 2 * h->M;
+    dim3 grid(h->nwi, (h->pz + 1) / 2);
+    ProfScope prof("tomo_gather", st);
+    hipLaunchKernelGGL(k_gather_tile, grid, dim3(256), sizeof(float4) * E * E, st, h->Q, h->Bg,
+                       h->spos, h->sfloor, h->wi_tile, h->wi_range, h->bin_sample, N, h->M, beta,
+                       h->ntile1, h->ntheta * N, h->pz);
+    PT_LAUNCH_CHECK();
+  }
+  {
     const double sgn = ((N / 2) % 2) ? -1.0 : 1.0;
     const float oscale = (float)(sgn * kPiRef / (double)h->mu / (double)N * (double)scale);
     dim3 grid(h->ntheta, h->pz);
     int thr = std::min(256, ((N + 63) / 64) * 64);
     if (!lds_fft_fits(h->plan, 1, thr)) thr = 256;
-    hipLaunchKernelGGL(k_gather_line, grid, dim3(thr), sizeof(float2) * N, st, g, h->Bg, h->spos,
-                       h->sfloor, h->tw, h->plan, h->M, beta, oscale, h->n, h->ntheta, h->pz);
+    ProfScope prof("tomo_line_idft", st);
+    hipLaunchKernelGGL(k_line_idft, grid, dim3(thr), sizeof(float2) * N, st, g, h->Q, h->tw,
+                       h->plan, oscale, h->n, h->ntheta, h->pz);
     PT_LAUNCH_CHECK();
   }
   return PT_OK;
@@ -428,30 +500,35 @@ int run_adj(pt_tomo* h, float2* f, const float2* g, float scale, hipStream_t st)
   {
     const double sgn = ((N / 2) % 2) ? -1.0 : 1.0;
     const float oscale = (float)(sgn * kPiRef / (double)h->mu / (double)N);
-    dim3 grid(h->ntheta, h->pz);
-    int thr = std::min(256, ((N + 63) / 64) * 64);
-    if (!lds_fft_fits(h->plan, 1, thr)) thr = 256;
-    hipLaunchKernelGGL(k_adj_line, grid, dim3(thr), sizeof(float2) * N, st, h->Q, g, h->tw,
-                       h->plan, oscale, h->n, h->ntheta, h->pz);
+    const int zq = h->adj_lines;
+    dim3 grid(h->ntheta, (h->pz + zq - 1) / zq);
+    ProfScope prof("tomo_adj_line", st);
+    hipLaunchKernelGGL(k_adj_line, grid, dim3(256), sizeof(float2) * zq * N, st, h->Q, g, h->tw,
+                       h->plan, oscale, h->n, h->ntheta, h->pz, zq);
     PT_LAUNCH_CHECK();
   }
   {
     const float beta = (float)(kPiRef * kPiRef / (double)h->mu);
-    const int E = kTile + 2 * M;
-    dim3 grid(h->nwi, h->pz);
-    hipLaunchKernelGGL(k_scatter, grid, dim3(64), sizeof(float2) * E * E, st, h->scratch, h->Q,
-                       h->spos, h->sfloor, h->wi_tile, h->wi_range, h->bin_sample, N, M, beta,
-                       h->ntile1, h->ntheta * N, h->pz);
-    PT_LAUNCH_CHECK();
-    long total = (long)h->pz * 4 * N * N;
-    hipLaunchKernelGGL(k_fold_grid, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, h->Bg,
-                       h->scratch, h->tile_wi, N, M, h->ntile1, h->pz);
-    PT_LAUNCH_CHECK();
+    dim3 grid(h->nswi, (h->pz + kZS - 1) / kZS);
+    {
+      ProfScope prof("tomo_scatter", st);
+      hipLaunchKernelGGL(k_scatter_own, grid, dim3(64), 0, st, h->Bg, h->spart, h->Q, h->spos,
+                         h->sfloor, h->ent_s, h->ent_o, h->swi_tile, h->swi_range, h->swi_slot, N,
+                         M, beta, h->nt8, h->pz);
+      PT_LAUNCH_CHECK();
+    }
+    if (h->nfix) {
+      ProfScope prof("tomo_scatter_fix", st);
+      hipLaunchKernelGGL(k_scatter_fix, dim3(h->nfix, h->pz), dim3(64), 0, st, h->Bg, h->spart,
+                         h->sfix, N, h->nt8, h->pz);
+      PT_LAUNCH_CHECK();
+    }
   }
   {
     const int cols = h->cols_per_wg;
     dim3 grid(2 * N / cols, h->pz);
     size_t lds = sizeof(float2) * 2 * cols * N;
+    ProfScope prof("tomo_col_adj", st);
     hipLaunchKernelGGL(k_col_adj, grid, dim3(nthr), lds, st, h->A, h->Bg, h->tw, h->twist, h->plan,
                        h->n, cols);
     PT_LAUNCH_CHECK();
@@ -460,6 +537,7 @@ int run_adj(pt_tomo* h, float2* f, const float2* g, float scale, hipStream_t st)
     const int rows = h->rows_per_wg;
     dim3 grid((h->n + rows - 1) / rows, h->pz);
     size_t lds = sizeof(float2) * 2 * rows * N;
+    ProfScope prof("tomo_row_adj", st);
     hipLaunchKernelGGL(k_row_adj, grid, dim3(nthr), lds, st, f, h->A, h->tw, h->twist, h->dphi,
                        h->mrange, h->plan, h->n, rows, scale);
     PT_LAUNCH_CHECK();
@@ -527,6 +605,7 @@ extern "C" int pt_tomo_create(int ntheta, int pnz, int n, pt_tomo** out) {
   int cols = 1;
   while (cols * 2 <= lines / 2 && (2 * N) % (cols * 2) == 0) cols *= 2;
   h->cols_per_wg = cols;
+  h->adj_lines = lines_fit(plan, 256, kZQ, cap);
   // host tables
   std::vector<float2> tw(N), twist(N);
   std::vector<float> dphi(N);
@@ -583,12 +662,30 @@ extern "C" int pt_tomo_create(int ntheta, int pnz, int n, pt_tomo** out) {
   return PT_OK;
 }
 
+static void free_bins(pt_tomo* h) {
+  void** ptrs[] = {(void**)&h->wi_tile, (void**)&h->wi_range, (void**)&h->ent_s,
+                   (void**)&h->ent_o, (void**)&h->swi_tile, (void**)&h->swi_range,
+                   (void**)&h->swi_slot, (void**)&h->sfix, (void**)&h->spart};
+  for (void** p : ptrs) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  h->nwi = h->nswi = h->nfix = 0;
+}
+
+template <typename T>
+static int upload(T** dst, const std::vector<T>& v, hipStream_t st) {
+  PT_HIP(hipMalloc(dst, sizeof(T) * std::max<size_t>(v.size(), 1)));
+  if (!v.empty()) PT_HIP(hipMemcpyAsync(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+  return PT_OK;
+}
+
 extern "C" int pt_tomo_destroy(pt_tomo* h) {
   if (!h) return PT_OK;
   (void)hipFree(h->tw); (void)hipFree(h->twist); (void)hipFree(h->dphi); (void)hipFree(h->mrange);
   (void)hipFree(h->spos); (void)hipFree(h->sfloor); (void)hipFree(h->A); (void)hipFree(h->Bg); (void)hipFree(h->Q);
-  (void)hipFree(h->wi_tile); (void)hipFree(h->wi_range); (void)hipFree(h->bin_sample); (void)hipFree(h->tile_wi);
-  (void)hipFree(h->scratch);
+  (void)hipFree(h->bin_sample);
+  free_bins(h);
   delete h;
   return PT_OK;
 }
@@ -619,55 +716,107 @@ extern "C" int pt_tomo_set_theta(pt_tomo* h, const float* theta, pt_stream strea
       fl[(long)t * N + i] = make_int2((int)std::floor(px), (int)std::floor(py));
     }
   }
-  // bins by kTile x kTile tile of the floor position
-  const int nt1 = (2 * N + kTile - 1) / kTile;
-  const long ntiles = (long)nt1 * nt1;
-  std::vector<int> cnt(ntiles + 1, 0);
-  for (long s = 0; s < ns; ++s) {
-    const int tyi = (fl[s].y + N) / kTile, txi = (fl[s].x + N) / kTile;
-    if (fl[s].x < -N || fl[s].x >= N || fl[s].y < -N || fl[s].y >= N) {
+  for (long q = 0; q < ns; ++q) {
+    if (fl[q].x < -N || fl[q].x >= N || fl[q].y < -N || fl[q].y >= N) {
       set_error("pt_tomo_set_theta: sample outside the gridding domain");
       return PT_ERR_ARG;
     }
-    cnt[(long)tyi * nt1 + txi + 1]++;
   }
+  const int M = h->M, twoNi = 2 * N;
+  // gather bins: samples by kTile x kTile tile of their floor position,
+  // chunks of at most kWorkMax samples
+  const int nt1 = (twoNi + kTile - 1) / kTile;
+  const long ntiles = (long)nt1 * nt1;
+  std::vector<int> cnt(ntiles + 1, 0);
+  for (long q = 0; q < ns; ++q)
+    cnt[(long)((fl[q].y + N) / kTile) * nt1 + (fl[q].x + N) / kTile + 1]++;
   for (long i = 0; i < ntiles; ++i) cnt[i + 1] += cnt[i];
   std::vector<int> fillp(cnt.begin(), cnt.end() - 1), list(ns);
-  for (long s = 0; s < ns; ++s) {
-    const int tyi = (fl[s].y + N) / kTile, txi = (fl[s].x + N) / kTile;
-    list[fillp[(long)tyi * nt1 + txi]++] = (int)s;
-  }
+  for (long q = 0; q < ns; ++q)
+    list[fillp[(long)((fl[q].y + N) / kTile) * nt1 + (fl[q].x + N) / kTile]++] = (int)q;
   std::vector<int> wtile;
   std::vector<int2> wrange;
-  std::vector<int> twi(ntiles + 1, 0);
-  for (long tI = 0; tI < ntiles; ++tI) {
-    twi[tI] = (int)wtile.size();
+  for (long tI = 0; tI < ntiles; ++tI)
     for (int s0 = cnt[tI]; s0 < cnt[tI + 1]; s0 += kWorkMax) {
       wtile.push_back((int)tI);
       wrange.push_back(make_int2(s0, std::min(s0 + kWorkMax, cnt[tI + 1])));
     }
+  // scatter lists: for every kTS x kTS grid tile, the samples whose 15 x 15
+  // footprint (periodic in 2N) covers part of it, in sample order, with the
+  // footprint origin relative to the tile in [-2M, kTS-1]
+  const int nt8 = (twoNi + kTS - 1) / kTS;
+  const long ntl8 = (long)nt8 * nt8;
+  auto tiles_of = [&](int origin, int* tl, int* off) {
+    int cntt = 0;
+    for (int k = 0; k <= 2 * M; ++k) {
+      int g = origin + k;
+      g %= twoNi;
+      if (g < 0) g += twoNi;
+      const int t = g / kTS;
+      bool seen = false;
+      for (int j = 0; j < cntt; ++j) seen |= (tl[j] == t);
+      if (seen) continue;
+      int o = (origin - t * kTS) % twoNi;
+      if (o < 0) o += twoNi;
+      if (o > kTS - 1) o -= twoNi;
+      tl[cntt] = t;
+      off[cntt] = o;
+      ++cntt;
+    }
+    return cntt;
+  };
+  std::vector<int> ecnt(ntl8 + 1, 0);
+  int tx[4], ox[4], ty[4], oy[4];
+  for (long q = 0; q < ns; ++q) {
+    const int nx = tiles_of(N + fl[q].x - M, tx, ox), ny = tiles_of(N + fl[q].y - M, ty, oy);
+    for (int a = 0; a < ny; ++a)
+      for (int b = 0; b < nx; ++b) ecnt[(long)ty[a] * nt8 + tx[b] + 1]++;
   }
-  twi[ntiles] = (int)wtile.size();
-  const int nwi = (int)wtile.size();
-  const int E = kTile + 2 * h->M;
-  (void)hipFree(h->wi_tile); (void)hipFree(h->wi_range); (void)hipFree(h->tile_wi); (void)hipFree(h->scratch);
-  h->wi_tile = nullptr; h->wi_range = nullptr; h->tile_wi = nullptr; h->scratch = nullptr;
+  for (long i = 0; i < ntl8; ++i) ecnt[i + 1] += ecnt[i];
+  std::vector<int> efill(ecnt.begin(), ecnt.end() - 1), es(ecnt[ntl8]), eo(ecnt[ntl8]);
+  for (long q = 0; q < ns; ++q) {
+    const int nx = tiles_of(N + fl[q].x - M, tx, ox), ny = tiles_of(N + fl[q].y - M, ty, oy);
+    for (int a = 0; a < ny; ++a)
+      for (int b = 0; b < nx; ++b) {
+        const long tI = (long)ty[a] * nt8 + tx[b];
+        const int at = efill[tI]++;
+        es[at] = (int)q;
+        eo[at] = (ox[b] + 64) | ((oy[a] + 64) << 8);
+      }
+  }
+  std::vector<int> stile, sslot;
+  std::vector<int2> srange;
+  std::vector<int4> sfix;
+  int nslots = 0;
+  for (long tI = 0; tI < ntl8; ++tI) {
+    const int e0 = ecnt[tI], e1 = ecnt[tI + 1];
+    const int nch = std::max(1, (e1 - e0 + kSChunk - 1) / kSChunk);
+    if (nch > 1) sfix.push_back(make_int4((int)tI, nslots, nch, 0));
+    for (int c = 0; c < nch; ++c) {
+      stile.push_back((int)tI);
+      srange.push_back(make_int2(std::min(e0 + c * kSChunk, e1), std::min(e0 + (c + 1) * kSChunk, e1)));
+      sslot.push_back(nch > 1 ? nslots + c : -1);
+    }
+    if (nch > 1) nslots += nch;
+  }
+  free_bins(h);
   h->ready = false;
-  PT_HIP(hipMalloc(&h->wi_tile, sizeof(int) * std::max(nwi, 1)));
-  PT_HIP(hipMalloc(&h->wi_range, sizeof(int2) * std::max(nwi, 1)));
-  PT_HIP(hipMalloc(&h->tile_wi, sizeof(int) * (ntiles + 1)));
-  PT_HIP(hipMalloc(&h->scratch, sizeof(float2) * (size_t)std::max(nwi, 1) * h->pz * E * E));
+  int rc;
+  if ((rc = upload(&h->wi_tile, wtile, st)) || (rc = upload(&h->wi_range, wrange, st)) ||
+      (rc = upload(&h->ent_s, es, st)) || (rc = upload(&h->ent_o, eo, st)) ||
+      (rc = upload(&h->swi_tile, stile, st)) || (rc = upload(&h->swi_range, srange, st)) ||
+      (rc = upload(&h->swi_slot, sslot, st)) || (rc = upload(&h->sfix, sfix, st)))
+    return rc;
+  PT_HIP(hipMalloc(&h->spart, sizeof(float2) * (size_t)std::max(nslots, 1) * h->pz * 64));
   PT_HIP(hipMemcpyAsync(h->spos, pos.data(), sizeof(float2) * ns, hipMemcpyHostToDevice, st));
   PT_HIP(hipMemcpyAsync(h->sfloor, fl.data(), sizeof(int2) * ns, hipMemcpyHostToDevice, st));
   PT_HIP(hipMemcpyAsync(h->bin_sample, list.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st));
-  if (nwi) {
-    PT_HIP(hipMemcpyAsync(h->wi_tile, wtile.data(), sizeof(int) * nwi, hipMemcpyHostToDevice, st));
-    PT_HIP(hipMemcpyAsync(h->wi_range, wrange.data(), sizeof(int2) * nwi, hipMemcpyHostToDevice, st));
-  }
-  PT_HIP(hipMemcpyAsync(h->tile_wi, twi.data(), sizeof(int) * (ntiles + 1), hipMemcpyHostToDevice, st));
   PT_HIP(hipStreamSynchronize(st));
   h->ntile1 = nt1;
-  h->nwi = nwi;
+  h->nwi = (int)wtile.size();
+  h->nt8 = nt8;
+  h->nswi = (int)stile.size();
+  h->nfix = (int)sfix.size();
   h->ready = true;
   return PT_OK;
 }

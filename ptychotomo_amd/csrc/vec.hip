@@ -5,6 +5,7 @@
 #include <cmath>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "common.hpp"
 
@@ -12,6 +13,39 @@ namespace pt {
 
 static thread_local std::string g_last_error = "no error";
 void set_error(const std::string& msg) { g_last_error = msg; }
+
+// ------------------------------------------------------------------ launch profiler
+namespace {
+struct Profiler {
+  std::mutex mu;
+  bool on = false;
+  std::string tag;
+  std::vector<hipEvent_t> ev;  // pairs (start, stop)
+  size_t used = 0;
+} g_prof;
+}  // namespace
+
+ProfScope::ProfScope(const char* tag, hipStream_t stream) : active(false), slot(-1), st(stream) {
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  if (!g_prof.on || g_prof.tag != tag) return;
+  if (g_prof.used + 2 > g_prof.ev.size()) {
+    for (int i = 0; i < 64; ++i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      g_prof.ev.push_back(e);
+    }
+  }
+  slot = (int)g_prof.used;
+  g_prof.used += 2;
+  if (hipEventRecord(g_prof.ev[slot], st) != hipSuccess) return;
+  active = true;
+}
+
+ProfScope::~ProfScope() {
+  if (!active) return;
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  (void)hipEventRecord(g_prof.ev[slot + 1], st);
+}
 
 namespace {
 
@@ -64,6 +98,15 @@ __global__ void k_finish(double* __restrict__ out, const double* __restrict__ pa
   out[seg * nv + k] = s;
 }
 
+__global__ void k_finish_n(double* __restrict__ out, const double* __restrict__ partial,
+                           int nblk, int nv, int nout) {
+  const int seg = blockIdx.x, k = threadIdx.x;
+  if (k >= nout) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[((long)seg * nblk + b) * nv + k];
+  out[seg * nout + k] = s;
+}
+
 template <int NV, typename F>
 int seg_reduce(double* out, long len, int nseg, F f, hipStream_t st) {
   PT_REQUIRE(nseg > 0 && len % nseg == 0, PT_ERR_SIZE, "reduction: nseg must divide len");
@@ -76,6 +119,25 @@ int seg_reduce(double* out, long len, int nseg, F f, hipStream_t st) {
   hipLaunchKernelGGL((k_seg_reduce<NV, F>), dim3(nblk, nseg), dim3(RB), 0, st, part, seglen, f);
   PT_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_finish, dim3(nseg), dim3(64), 0, st, out, part, nblk, NV);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+// as seg_reduce with NV = PT_MAX_CAND accumulators, of which the first
+// `nout` are written per segment
+template <typename F>
+int seg_reduce_n(double* out, long len, int nseg, int nout, F f, hipStream_t st) {
+  constexpr int NV = 8;
+  PT_REQUIRE(nseg > 0 && len % nseg == 0, PT_ERR_SIZE, "reduction: nseg must divide len");
+  const long seglen = len / nseg;
+  int nblk = (int)std::min<long>(RMAXBLK, std::max<long>(1, (seglen + RB * 4 - 1) / (RB * 4)));
+  nblk = std::max(1, std::min(nblk, std::max(1, 2048 / nseg)));
+  double* part;
+  int rc = partial_buf((size_t)nseg * nblk * NV, &part);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_seg_reduce<NV, F>), dim3(nblk, nseg), dim3(RB), 0, st, part, seglen, f);
+  PT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_finish_n, dim3(nseg), dim3(64), 0, st, out, part, nblk, NV, nout);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -307,17 +369,6 @@ struct FTomoResid {
     v[0] += (double)e.x * e.x + (double)e.y * e.y;
   }
 };
-// cg_tomo line-search terms: w = K*Rd; v[0] += Re conj(K*Ru - xi0)*w, v[1] += |w|^2
-struct FTomoLs {
-  const float2* K; const float2* Ru; const float2* Rd; const float2* xi0;
-  __device__ void operator()(long i, double (&v)[2]) const {
-    const float2 k = K[i];
-    const float2 e = csub(cmul(k, Ru[i]), xi0[i]);
-    const float2 w = cmul(k, Rd[i]);
-    v[0] += (double)e.x * w.x + (double)e.y * w.y;
-    v[1] += (double)w.x * w.x + (double)w.y * w.y;
-  }
-};
 
 __device__ __forceinline__ void grad3(const float2* __restrict__ u, long i, int x, int y, int z,
                                       int nz, int ny, int nx, float2 (&g)[3]) {
@@ -366,35 +417,73 @@ struct FRegGrad {
     grad[i] = make_float2(gv.x + c * s.x, gv.y + c * s.y);
   }
 };
-// regularisation line-search terms: v[0] += Re conj(J u - xi1)*J d, v[1] += |J d|^2
-struct FRegLs {
-  const float2* u; const float2* d; const float2* xi1; int nz, ny, nx;
-  __device__ void operator()(long i, double (&v)[2]) const {
+// Line-search candidates, evaluated as the reference forms them: the
+// stepped quantity x + gamma*dx element-wise in FP32 (a step below FP32
+// resolution gives a bit-identical term), accumulation in FP64.
+#define PT_MAX_CAND 8
+// cg_ptycho rho term (solver.py:213): |h - (psi + g*d) + lamd/rho|^2
+struct FRhoMinf {
+  const float2* h; const float2* psi; const float2* lamd; const float2* d; float rho;
+  const double* gam; int ngam; long seglen;
+  __device__ void operator()(long i, double (&v)[PT_MAX_CAND]) const {
+    const long s = i / seglen;
+    const float2 a = h[i], p = psi[i], l = lamd[i], dd = d[i];
+    const float lx = l.x / rho, ly = l.y / rho;
+#pragma unroll
+    for (int j = 0; j < PT_MAX_CAND; ++j) {
+      if (j < ngam) {
+        const float g = (float)gam[s * ngam + j];
+        const float tx = p.x + g * dd.x, ty = p.y + g * dd.y;
+        const float rx = a.x - tx + lx, ry = a.y - ty + ly;
+        v[j] += (double)(rx * rx + ry * ry);
+      }
+    }
+  }
+};
+// cg_tomo data term (solver.py:182-183): |K*Ru + g*(K*Rd) - xi0|^2
+struct FTomoMinf {
+  const float2* K; const float2* Ru; const float2* Rd; const float2* xi0; const double* gam;
+  int ngam;
+  __device__ void operator()(long i, double (&v)[PT_MAX_CAND]) const {
+    const float2 k = K[i];
+    const float2 a = cmul(k, Ru[i]), b = cmul(k, Rd[i]), x = xi0[i];
+#pragma unroll
+    for (int j = 0; j < PT_MAX_CAND; ++j) {
+      if (j < ngam) {
+        const float g = (float)gam[j];
+        const float rx = (a.x + g * b.x) - x.x, ry = (a.y + g * b.y) - x.y;
+        v[j] += (double)(rx * rx + ry * ry);
+      }
+    }
+  }
+};
+// cg_tomo regulariser term: |J u + g*J d - xi1|^2 over the 3 components
+struct FRegMinf {
+  const float2* u; const float2* d; const float2* xi1; const double* gam; int ngam;
+  int nz, ny, nx;
+  __device__ void operator()(long i, double (&v)[PT_MAX_CAND]) const {
     const long plane = (long)nz * ny * nx;
     const int x = (int)(i % nx);
     const long r = i / nx;
     const int y = (int)(r % ny);
     const int z = (int)(r / ny);
-    float2 gu[3], gd[3];
+    float2 gu[3], gd[3], xi[3];
     grad3(u, i, x, y, z, nz, ny, nx, gu);
     grad3(d, i, x, y, z, nz, ny, nx, gd);
-    for (int k = 0; k < 3; ++k) {
-      const float2 e = csub(gu[k], xi1[k * plane + i]);
-      v[0] += (double)e.x * gd[k].x + (double)e.y * gd[k].y;
-      v[1] += (double)gd[k].x * gd[k].x + (double)gd[k].y * gd[k].y;
+    for (int k = 0; k < 3; ++k) xi[k] = xi1[k * plane + i];
+#pragma unroll
+    for (int j = 0; j < PT_MAX_CAND; ++j) {
+      if (j < ngam) {
+        const float g = (float)gam[j];
+        double dacc = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          const float rx = (gu[k].x + g * gd[k].x) - xi[k].x;
+          const float ry = (gu[k].y + g * gd[k].y) - xi[k].y;
+          dacc += (double)(rx * rx + ry * ry);
+        }
+        v[j] += dacc;
+      }
     }
-  }
-};
-// cg_ptycho rho terms per chunk with r0 = h - psi + lamd/rho:
-// v[0] += |r0|^2, v[1] += Re conj(r0)*d, v[2] += |d|^2
-struct FRhoTerms {
-  const float2* h; const float2* psi; const float2* lamd; const float2* d; float rho;
-  __device__ void operator()(long i, double (&v)[3]) const {
-    const float2 a = h[i], p = psi[i], l = lamd[i], dd = d[i];
-    const float2 r0 = make_float2(a.x - p.x + l.x / rho, a.y - p.y + l.y / rho);
-    v[0] += (double)r0.x * r0.x + (double)r0.y * r0.y;
-    v[1] += (double)r0.x * dd.x + (double)r0.y * dd.y;
-    v[2] += (double)dd.x * dd.x + (double)dd.y * dd.y;
   }
 };
 // sum conj(a)*(b - c)
@@ -619,36 +708,12 @@ extern "C" int pt_k_tomo_resid(void* r, double* out, const void* K, const void* 
                                   (const float2*)xi0},
                        (hipStream_t)stream);
 }
-extern "C" int pt_k_tomo_ls(double* out, const void* K, const void* Ru, const void* Rd,
-                            const void* xi0, long len, pt_stream stream) {
-  PT_REQUIRE(out && K && Ru && Rd && xi0, PT_ERR_ARG, "pt_k_tomo_ls: null argument");
-  return seg_reduce<2>(out, len, 1,
-                       FTomoLs{(const float2*)K, (const float2*)Ru, (const float2*)Rd,
-                               (const float2*)xi0},
-                       (hipStream_t)stream);
-}
 extern "C" int pt_k_reg_grad(void* grad, double* out, const void* u, const void* xi1, double tau,
                              int nz, int ny, int nx, pt_stream stream) {
   PT_REQUIRE(grad && out && u && xi1, PT_ERR_ARG, "pt_k_reg_grad: null argument");
   return seg_reduce<1>(out, (long)nz * ny * nx, 1,
                        FRegGrad{(float2*)grad, (const float2*)u, (const float2*)xi1, (float)tau,
                                 nz, ny, nx},
-                       (hipStream_t)stream);
-}
-extern "C" int pt_k_reg_ls(double* out, const void* u, const void* d, const void* xi1, int nz,
-                           int ny, int nx, pt_stream stream) {
-  PT_REQUIRE(out && u && d && xi1, PT_ERR_ARG, "pt_k_reg_ls: null argument");
-  return seg_reduce<2>(out, (long)nz * ny * nx, 1,
-                       FRegLs{(const float2*)u, (const float2*)d, (const float2*)xi1, nz, ny, nx},
-                       (hipStream_t)stream);
-}
-extern "C" int pt_k_rho_terms(double* out, const void* h, const void* psi, const void* lamd,
-                              const void* d, double rho, long len, int nseg, pt_stream stream) {
-  PT_REQUIRE(out && h && psi && lamd && d, PT_ERR_ARG, "pt_k_rho_terms: null argument");
-  PT_REQUIRE(rho > 0, PT_ERR_ARG, "pt_k_rho_terms: rho must be positive");
-  return seg_reduce<3>(out, len, nseg,
-                       FRhoTerms{(const float2*)h, (const float2*)psi, (const float2*)lamd,
-                                 (const float2*)d, (float)rho},
                        (hipStream_t)stream);
 }
 extern "C" int pt_k_dot_diff(double* out, const void* a, const void* b, const void* c, long len,
@@ -706,4 +771,65 @@ struct FIntensity {
 extern "C" int pt_k_intensity(float* out, const void* g, float scale, long len, pt_stream stream) {
   PT_REQUIRE(out && g, PT_ERR_ARG, "pt_k_intensity: null argument");
   return map(len, FIntensity{out, (const float2*)g, scale}, (hipStream_t)stream);
+}
+
+extern "C" int pt_prof_start(const char* tag) {
+  PT_REQUIRE(tag, PT_ERR_ARG, "pt_prof_start: null tag");
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  g_prof.on = true;
+  g_prof.tag = tag;
+  g_prof.used = 0;
+  return PT_OK;
+}
+
+extern "C" int pt_prof_stop(double* total_ms, long* launches) {
+  PT_REQUIRE(total_ms && launches, PT_ERR_ARG, "pt_prof_stop: null argument");
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  g_prof.on = false;
+  double tot = 0.0;
+  long cnt = 0;
+  for (size_t i = 0; i + 1 < g_prof.used; i += 2) {
+    PT_HIP(hipEventSynchronize(g_prof.ev[i + 1]));
+    float ms = 0.f;
+    PT_HIP(hipEventElapsedTime(&ms, g_prof.ev[i], g_prof.ev[i + 1]));
+    tot += ms;
+    ++cnt;
+  }
+  g_prof.used = 0;
+  *total_ms = tot;
+  *launches = cnt;
+  return PT_OK;
+}
+
+extern "C" int pt_k_rho_minf(double* out, const void* h, const void* psi, const void* lamd,
+                             const void* d, double rho, const double* gammas, int ngam, long len,
+                             int nseg, pt_stream stream) {
+  PT_REQUIRE(out && h && psi && lamd && d && gammas, PT_ERR_ARG, "pt_k_rho_minf: null argument");
+  PT_REQUIRE(rho > 0, PT_ERR_ARG, "pt_k_rho_minf: rho must be positive");
+  PT_REQUIRE(ngam >= 1 && ngam <= PT_MAX_CAND, PT_ERR_ARG, "pt_k_rho_minf: 1 <= ngam <= 8");
+  PT_REQUIRE(nseg > 0 && len % nseg == 0, PT_ERR_SIZE, "pt_k_rho_minf: nseg must divide len");
+  return seg_reduce_n(out, len, nseg, ngam,
+                      FRhoMinf{(const float2*)h, (const float2*)psi, (const float2*)lamd,
+                               (const float2*)d, (float)rho, gammas, ngam, len / nseg},
+                      (hipStream_t)stream);
+}
+extern "C" int pt_k_tomo_minf(double* out, const void* K, const void* Ru, const void* Rd,
+                              const void* xi0, const double* gammas, int ngam, long len,
+                              pt_stream stream) {
+  PT_REQUIRE(out && K && Ru && Rd && xi0 && gammas, PT_ERR_ARG, "pt_k_tomo_minf: null argument");
+  PT_REQUIRE(ngam >= 1 && ngam <= PT_MAX_CAND, PT_ERR_ARG, "pt_k_tomo_minf: 1 <= ngam <= 8");
+  return seg_reduce_n(out, len, 1, ngam,
+                      FTomoMinf{(const float2*)K, (const float2*)Ru, (const float2*)Rd,
+                                (const float2*)xi0, gammas, ngam},
+                      (hipStream_t)stream);
+}
+extern "C" int pt_k_reg_minf(double* out, const void* u, const void* d, const void* xi1,
+                             const double* gammas, int ngam, int nz, int ny, int nx,
+                             pt_stream stream) {
+  PT_REQUIRE(out && u && d && xi1 && gammas, PT_ERR_ARG, "pt_k_reg_minf: null argument");
+  PT_REQUIRE(ngam >= 1 && ngam <= PT_MAX_CAND, PT_ERR_ARG, "pt_k_reg_minf: 1 <= ngam <= 8");
+  return seg_reduce_n(out, (long)nz * ny * nx, 1, ngam,
+                      FRegMinf{(const float2*)u, (const float2*)d, (const float2*)xi1, gammas,
+                               ngam, nz, ny, nx},
+                      (hipStream_t)stream);
 }

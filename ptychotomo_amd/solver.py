@@ -12,12 +12,14 @@ in rounding only):
   whole volume in one native call; the angle chunking of ``ptheta`` is kept
   where it changes the mathematics (per-chunk CG step and Dai-Yuan beta,
   solver.py:245-254), as segmented reductions.
-* ``cg_tomo`` evaluates its line-search functional as the exact quadratic
-  in gamma (3 reductions) instead of two full re-evaluations per trial, and
-  updates R u by linearity (R(u + g d) = R u + g R d) instead of a fresh
-  transform; the halving rule of ``line_search`` is applied unchanged.
-* ``cg_ptycho`` evaluates up to 8 halving candidates per sweep in one fused
-  far-field pass; the rho term is the exact quadratic in gamma.
+* ``cg_tomo`` updates R u by linearity (R(u + g d) = R u + g R d) instead of
+  a fresh transform per iteration.
+* The line searches evaluate up to 7 halving candidates per sweep in one
+  fused pass (``cg_ptycho``: far field and rho term; ``cg_tomo``: K R and J
+  terms).  Each candidate is formed as the reference forms it -- x + g*dx
+  element-wise in FP32 -- so a step below FP32 resolution reproduces the
+  gamma = 0 value exactly and the halving rule stops where the reference's
+  does instead of collapsing to gamma = 0.
 * Scalars (norms, dots, steps) are accumulated in FP64.
 """
 import ctypes
@@ -40,6 +42,19 @@ _MAXGAM = 8
 
 def _call(name, *args):
     return _lib.call(name, *args)
+
+
+def _dai_yuan(gnorm2, den):
+    """Dai-Yuan beta = ||g||^2 / sum(conj(d)(g - g0)) (solver.py:195-196,
+    230-231, complex).  After a zero step (the halving rule found no descent)
+    g == g0 and the reference divides 0 by 0; here beta = 0 instead, i.e. a
+    steepest-descent restart, so the iterate stays finite."""
+    den = np.asarray(den, np.complex128)
+    ok = (den != 0) & np.isfinite(den)
+    beta = np.zeros(den.shape, np.complex128)
+    beta[ok] = np.asarray(gnorm2, np.float64)[ok] / den[ok]
+    beta[~np.isfinite(beta)] = 0
+    return beta
 
 
 class Solver(object):
@@ -253,14 +268,32 @@ class Solver(object):
         return gamma
 
     @staticmethod
-    def _halving(gamma, dec):
-        """Apply the halving rule of line_search given dec(g) = minf(0) - minf(g).
-        Returns (gamma, number of evaluations)."""
-        while dec(gamma) < 0 and gamma > 1e-12:
-            gamma *= 0.5
-        if gamma <= 1e-12:
-            gamma = 0
-        return gamma
+    def _halving_sweeps(gamma, evalf):
+        """Halving rule of line_search (solver.py:171-177) for a vector of
+        independent segments.  ``evalf(cand)`` returns the functional at the
+        candidate steps cand[s, j] (column 0 is gamma = 0); each sweep tries
+        the current step and its next 6 halvings for every segment."""
+        nseg = len(gamma)
+        cur = np.array(gamma, dtype=np.float64)
+        out = np.zeros(nseg)
+        done = np.zeros(nseg, bool)
+        while not done.all():
+            cand = np.zeros((nseg, _MAXGAM))
+            cand[:, 1:] = cur[:, None] * 0.5 ** np.arange(_MAXGAM - 1)[None, :]
+            f = evalf(cand)
+            for s in range(nseg):
+                if done[s]:
+                    continue
+                for j in range(1, _MAXGAM):
+                    g = cand[s, j]
+                    if (f[s, 0] - f[s, j] < 0) and g > 1e-12:
+                        continue
+                    out[s] = 0.0 if g <= 1e-12 else g
+                    done[s] = True
+                    break
+                else:
+                    cur[s] = cand[s, -1] * 0.5
+        return out
 
     # ------------------------------------------------------------------ cg_tomo
     def cg_tomo(self, xi0, xi1, K, init, rho, tau, titer):
@@ -286,6 +319,7 @@ class Solver(object):
         gamma = 2.0
         gammas = []
         beta_dev = torch.zeros(2, dtype=f64, device=device())
+        gam_dev = torch.zeros(_MAXGAM, dtype=f64, device=device())
         for i in range(titer):
             # grad = rho*R*(conj(K)(K Ru - xi0)) + tau*J*(J u - xi1)
             _call("pt_k_tomo_resid", ptr(r_t), ptr(sc[0:]), ptr(K), ptr(Ru), ptr(xi0), lenp, st)
@@ -297,18 +331,22 @@ class Solver(object):
             else:
                 _call("pt_k_dy_reduce", ptr(sc[2:]), ptr(grad), ptr(grad0), ptr(d), lenu, 1, st)
                 gn, dr, di = self._scalars(5)[2:5]
-                beta = gn / complex(dr, di)
+                beta = complex(_dai_yuan(np.array([gn]), np.array([complex(dr, di)]))[0])
                 beta_dev.copy_(torch.tensor([beta.real, beta.imag], dtype=f64))
                 _call("pt_k_dy_update", ptr(d), ptr(grad), ptr(beta_dev), 0, lenu, 1, st)
             grad0 = grad
             Rd = self.fwd_tomo_batch(d)
-            _call("pt_k_tomo_ls", ptr(sc[8:]), ptr(K), ptr(Ru), ptr(Rd), ptr(xi0), lenp, st)
-            _call("pt_k_reg_ls", ptr(sc[10:]), ptr(u), ptr(d), ptr(xi1), nz, n, n, st)
-            v = self._scalars(12)
-            a1 = 2.0 * (rho * v[8] + tau * v[10])
-            a2 = rho * v[9] + tau * v[11]
-            # minf(u) - minf(u + g d) = -(a1 g + a2 g^2)
-            gamma = self._halving(gamma, lambda g: -(a1 * g + a2 * g * g))
+
+            def evalf(cand):
+                gam_dev.copy_(torch.from_numpy(cand[0]))
+                _call("pt_k_tomo_minf", ptr(sc[16:]), ptr(K), ptr(Ru), ptr(Rd), ptr(xi0),
+                      ptr(gam_dev), _MAXGAM, lenp, st)
+                _call("pt_k_reg_minf", ptr(sc[24:]), ptr(u), ptr(d), ptr(xi1), ptr(gam_dev),
+                      _MAXGAM, nz, n, n, st)
+                v = self._scalars(32)
+                return (rho * v[16:24] + tau * v[24:32])[None, :]
+
+            gamma = float(self._halving_sweeps([gamma], evalf)[0])
             gammas.append(gamma)
             _call("pt_k_lincomb3", ptr(u), 1.0, 0.0, ptr(u), gamma, 0.0, ptr(d), 0.0, 0.0, None,
                   lenu, st)
@@ -342,6 +380,7 @@ class Solver(object):
         L = psi.numel()
         sc = torch.zeros(8 * nseg + 8, dtype=f64, device=device())
         sums = torch.zeros(nseg * _MAXGAM, dtype=f64, device=device())
+        rsums = torch.zeros(nseg * _MAXGAM, dtype=f64, device=device())
         gam_dev = torch.zeros(nseg * _MAXGAM, dtype=f64, device=device())
         beta_dev = torch.zeros(2 * nseg, dtype=f64, device=device())
         step_dev = torch.zeros(2 * nseg, dtype=f64, device=device())
@@ -360,16 +399,22 @@ class Solver(object):
             else:
                 _call("pt_k_dy_reduce", ptr(sc), ptr(grad), ptr(grad0), ptr(d), L, nseg, st)
                 v = sc[:3 * nseg].cpu().numpy().reshape(nseg, 3)
-                beta = v[:, 0] / (v[:, 1] + 1j * v[:, 2])
+                beta = _dai_yuan(v[:, 0], v[:, 1] + 1j * v[:, 2])
                 beta_dev.copy_(torch.from_numpy(np.stack([beta.real, beta.imag], 1).ravel()))
                 _call("pt_k_dy_update", ptr(d), ptr(grad), ptr(beta_dev), 0, L, nseg, st)
             grad, grad0 = grad0, grad
-            # rho term: rho*||r0 - g d||^2 = A - 2 g B + g^2 C  (r0 = h - psi + lamd/rho)
-            _call("pt_k_rho_terms", ptr(sc), ptr(h), ptr(psi), ptr(lamd), ptr(d), float(rho), L,
-                  nseg, st)
-            rt = sc[:3 * nseg].cpu().numpy().reshape(nseg, 3)
-            gamma = self._ptycho_line_search(op, nseg, chunk, psi, d, data, gamma, rho, rt, mdl,
-                                             cp, cd, sums, gam_dev)
+
+            def evalf(cand):
+                gam_dev.copy_(torch.from_numpy(cand.ravel()))
+                _call("pt_ptycho_minf", op._h, ptr(sums), ptr(psi), ptr(d), ptr(data),
+                      ptr(gam_dev), _MAXGAM, chunk, cp, cd, mdl, 0, st)
+                _call("pt_k_rho_minf", ptr(rsums), ptr(h), ptr(psi), ptr(lamd), ptr(d),
+                      float(rho), ptr(gam_dev), _MAXGAM, L, nseg, st)
+                dat = sums.cpu().numpy().reshape(nseg, _MAXGAM)
+                rr = rsums.cpu().numpy().reshape(nseg, _MAXGAM)
+                return dat + rho * rr
+
+            gamma = self._halving_sweeps(gamma, evalf)
             for s in range(nseg):
                 gam_hist[s].append(float(gamma[s]))
             step_dev.copy_(torch.from_numpy(np.stack([gamma, np.zeros(nseg)], 1).ravel()))
@@ -380,39 +425,6 @@ class Solver(object):
             print('possible phase wrap, max computed angle', mx)
         self._chunk_gammas = gam_hist
         return psi
-
-    def _ptycho_line_search(self, op, nseg, chunk, psi, d, data, gamma, rho, rt, mdl, cp, cd,
-                            sums, gam_dev):
-        """Halving rule of line_search per chunk; each sweep evaluates the
-        data term at gamma = 0 and up to 7 halving candidates per chunk."""
-        cur = gamma.copy()
-        done = np.zeros(nseg, bool)
-        f0 = None
-        out = np.zeros(nseg)
-        while not done.all():
-            cand = np.zeros((nseg, _MAXGAM))
-            for s in range(nseg):
-                cand[s, 1:] = cur[s] * 0.5 ** np.arange(_MAXGAM - 1)
-            gam_dev.copy_(torch.from_numpy(cand.ravel()))
-            _call("pt_ptycho_minf", op._h, ptr(sums), ptr(psi), ptr(d), ptr(data), ptr(gam_dev),
-                  _MAXGAM, chunk, cp, cd, mdl, 0, stream())
-            dat = sums.cpu().numpy().reshape(nseg, _MAXGAM)
-            for s in range(nseg):
-                if done[s]:
-                    continue
-                A, B, C = rho * rt[s, 0], rho * rt[s, 1], rho * rt[s, 2]
-                base = dat[s, 0] + A
-                for j in range(1, _MAXGAM):
-                    g = cand[s, j]
-                    fg = dat[s, j] + A - 2.0 * g * B + g * g * C
-                    if (base - fg < 0) and g > 1e-12:
-                        continue
-                    out[s] = 0.0 if g <= 1e-12 else g
-                    done[s] = True
-                    break
-                else:
-                    cur[s] = cand[s, -1] * 0.5
-        return out
 
     # ------------------------------------------------------------------ penalties / Lagrangian
     def update_penalty(self, psi, h, h0, phi, e, e0, rho, tau):
@@ -467,17 +479,22 @@ class Solver(object):
 
     # ------------------------------------------------------------------ ADMM
     def admm(self, data, h, e, psi, phi, lamd, mu, u, alpha, piter, titer, niter, model,
-             lagr_every=10, verbose=True, rho=0.5, tau=0.5):
-        """solver.py:308-336.  Returns (u, psi) like the reference; the full
-        final state (h, e, psi, phi, lamd, mu, u, rho, tau) is kept in
-        ``self.state`` and a per-iteration record in ``self.history``."""
+             lagr_every=10, verbose=True, rho=0.5, tau=0.5, m0=0, Ru=None, callback=None):
+        """solver.py:308-336.  Returns (u, psi) like the reference.
+
+        Beyond the reference: the full final state (h, e, psi, phi, lamd, mu,
+        u, rho, tau, Ru) is kept in ``self.state``; ``rho``, ``tau``, ``m0``
+        (index of the first iteration, which sets the piter+32 schedule of
+        the first two iterations and the Lagrangian cadence) and ``Ru``
+        (R u of the initial u) resume a run exactly where ``self.state`` left
+        it.  ``callback(m, solver_locals)`` is called after every iteration.
+        """
         data = self._dev_data(data)
         h, e, psi = to_dev(h, c64), to_dev(e, c64), to_dev(psi, c64)
         phi, lamd, mu, u = to_dev(phi, c64), to_dev(lamd, c64), to_dev(mu, c64), to_dev(u, c64)
         st = stream()
-        Ru = None
         self.history = []
-        for m in range(niter):
+        for m in range(m0, m0 + niter):
             h0, e0 = h, e
             psi = self.cg_ptycho_batch(data, psi, h, lamd, rho, piter + (m < 2) * 32, model)
             xi0, xi1, K, pshift = self.takexi(psi, phi, lamd, mu, rho, tau)
@@ -506,6 +523,16 @@ class Solver(object):
                     print("%d/%d) rho=%.2e, tau=%.2e, Lagrangian terms:  %.2e %.2e %.2e %.2e %.2e %.2e, Sum: %.2e" %
                           (m, niter, rho, tau, *lagr))
             self.history.append(rec)
+            if callback is not None:
+                callback(m, dict(h=h, e=e, psi=psi, phi=phi, lamd=lamd, mu=mu, u=u, rho=rho,
+                                 tau=tau, Ru=Ru))
         self.state = dict(h=h, e=e, psi=psi, phi=phi, lamd=lamd, mu=mu, u=u, rho=rho, tau=tau,
-                          Ru=Ru)
+                          Ru=Ru, m=m0 + niter)
         return u, psi
+
+    def admm_resume(self, data, alpha, piter, titer, niter, model, **kw):
+        """Continue from ``self.state`` (checkpoint/resume of the full state)."""
+        s = self.state
+        return self.admm(data, s["h"], s["e"], s["psi"], s["phi"], s["lamd"], s["mu"], s["u"],
+                         alpha, piter, titer, niter, model, rho=s["rho"], tau=s["tau"],
+                         m0=s["m"], Ru=s["Ru"], **kw)

@@ -1,0 +1,111 @@
+"""Seeded synthetic workloads of the benchmark configurations, generated on
+the device with the library's own forward operators.
+
+Data path (reference ``test.py:65-68`` with ``solver.py:108-117``):
+    data = Poisson(|F Q exptomo(R u)|^2 / coefdata)
+kept on the GPU.  The probe amplitude is scaled so that the brightest
+detector pixel receives ``photons`` expected counts before noise.
+
+Seeds: phantom = s, scan jitter = s+1, Poisson = s+2.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import objects
+from ._dev import c64, f32
+
+
+@dataclass
+class Workload:
+    name: str
+    n: int
+    nz: int
+    ntheta: int
+    nellipse: int
+    nprb: int = 16
+    sigma: float = 4.0
+    det: int = 16
+    step: int = 4
+    jitter: float = 1.0
+    photons: float = 1e3
+    ptheta: int = 32
+    pnz: int = 0            # 0 -> nz
+    piter: int = 4
+    titer: int = 4
+    alpha: float = 3e-7
+    voxelsize: float = 1e-6
+    energy: float = 8.8
+    model: str = "poisson"
+    seed: int = 0
+    niter: int = 32
+    semi: tuple = field(default=(4.0, 20.0))
+
+    def describe(self):
+        S = ((self.nz - self.nprb) // self.step + 1) * ((self.n - self.nprb) // self.step + 1)
+        return dict(workload=self.name, n=self.n, nz=self.nz, ntheta=self.ntheta,
+                    probe=self.nprb, det=self.det, step=self.step, jitter=self.jitter,
+                    nscan=S, photons=self.photons, ptheta=self.ptheta, piter=self.piter,
+                    titer=self.titer, model=self.model, ellipsoids=self.nellipse,
+                    seed=self.seed)
+
+
+# BASELINE configurations (the first is the CPU-runnable case, the second the
+# production single-GPU case, the last the large one)
+CONFIGS = {
+    "plain64": Workload("plain ADMM 64^3, 96 angles, 16x16 probe, step 4", 64, 64, 96, 20,
+                        seed=0, niter=32),
+    "prod256": Workload("plain ADMM 256^3, 384 angles, 16x16 probe, step 4", 256, 256, 384, 60,
+                        seed=1, niter=20),
+    "large512": Workload("plain ADMM 512^3, 768 angles, 32x32 probe, step 8", 512, 512, 768, 200,
+                         nprb=32, sigma=8.0, det=32, step=8, jitter=2.0, photons=1e2, seed=4,
+                         niter=20),
+}
+
+
+def make_inputs(w: Workload):
+    """Host-side geometry: theta, unit-peak probe, scan table, phantom."""
+    theta = objects.make_theta(w.ntheta)
+    prb = objects.gaussian_probe(w.nprb, w.sigma)
+    scan = objects.jittered_raster(w.ntheta, w.nz, w.n, w.nprb, w.step, w.jitter,
+                                   np.random.default_rng(w.seed + 1))
+    obj = objects.ellipsoid_phantom(w.nz, w.n, w.nellipse, np.random.default_rng(w.seed),
+                                    voxelsize=w.voxelsize, energy=w.energy, semi=w.semi)
+    return theta, prb, scan, obj
+
+
+def make_workload(w: Workload, noise=True):
+    """Build (solver, data, obj) for workload ``w``; data stays on the GPU."""
+    from .solver import Solver
+    theta, prb, scan, obj = make_inputs(w)
+    pnz = w.pnz or w.nz
+    args = (prb, scan, theta, [w.det, w.det], w.voxelsize, w.energy, w.ntheta, w.nz, w.n,
+            w.ptheta, pnz)
+    slv = Solver(*args)
+    objd = torch.from_numpy(obj).cuda()
+    data = slv.fwd_ptycho_batch_dev(slv.exptomo(slv.fwd_tomo_batch(objd)))
+    # data scales as |prb|^2: rescale the probe to the requested peak count
+    mx = float(data.max().item())
+    s = np.sqrt(w.photons / mx)
+    prb = (prb * np.float32(s)).astype(np.complex64)
+    slv = Solver(prb, *args[1:])
+    data = slv.fwd_ptycho_batch_dev(slv.exptomo(slv.fwd_tomo_batch(objd)))
+    if noise:
+        g = torch.Generator(device=data.device)
+        g.manual_seed(w.seed + 2)
+        data = torch.poisson(data, generator=g)
+    return slv, data.contiguous(), objd
+
+
+def initial_state(w: Workload):
+    """Reference initial guess (test.py:70-76): h = psi = 1, others 0."""
+    dev = torch.device("cuda")
+    sh = (w.ntheta, w.nz, w.n)
+    sh3 = (3, w.nz, w.n, w.n)
+    return dict(h=torch.ones(sh, dtype=c64, device=dev), e=torch.zeros(sh3, dtype=c64, device=dev),
+                psi=torch.ones(sh, dtype=c64, device=dev),
+                phi=torch.zeros(sh3, dtype=c64, device=dev),
+                lamd=torch.zeros(sh, dtype=c64, device=dev),
+                mu=torch.zeros(sh3, dtype=c64, device=dev),
+                u=torch.zeros((w.nz, w.n, w.n), dtype=c64, device=dev))

@@ -1,0 +1,70 @@
+"""CPU checks of the C ABI: the shared library loads, exports every
+function declared in include/ptychotomo_hip.h, and rejects invalid sizes and
+null arguments with the documented status codes (no GPU work is issued)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ptychotomo_hip.h")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    so = os.path.join(ROOT, "ptychotomo_amd", "libptychotomo_hip.so")
+    if not os.path.exists(so):
+        from ptychotomo_amd.build import build
+        build()
+    from ptychotomo_amd import _lib
+    return _lib
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pt_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_symbols_exported(lib):
+    names = declared()
+    assert len(names) > 30
+    missing = [n for n in names if not hasattr(lib.lib, n)]
+    assert not missing, missing
+    # the ctypes binding covers every declared entry point
+    assert set(names) <= set(lib.EXPORTED), set(names) - set(lib.EXPORTED)
+
+
+def test_size_preconditions(lib):
+    h = ctypes.c_void_p()
+    assert lib.lib.pt_tomo_create(10, 4, 30, ctypes.byref(h)) == -2      # n % 4
+    assert lib.lib.pt_tomo_create(10, 4, 44, ctypes.byref(h)) == -2      # 3n/2 = 66 has 11
+    assert lib.lib.pt_tomo_create(0, 4, 32, ctypes.byref(h)) == -2
+    assert lib.lib.pt_ptycho_create(4, 32, 32, 4, 9, 12, 12, 8, ctypes.byref(h)) == -2  # det 12
+    assert lib.lib.pt_ptycho_create(4, 32, 32, 4, 9, 16, 8, 8, ctypes.byref(h)) == -2   # detx != dety
+    assert lib.lib.pt_ptycho_create(4, 32, 32, 4, 9, 16, 16, 20, ctypes.byref(h)) == -2  # prb > det
+    assert b"probe" in lib.lib.pt_last_error()
+
+
+def test_null_arguments(lib):
+    assert lib.lib.pt_tomo_fwd(None, None, None, None) == -1
+    assert lib.lib.pt_ptycho_adj(None, None, None, None) == -1
+    assert lib.lib.pt_k_dot(None, None, None, 10, 1, None) == -1
+    assert lib.lib.pt_tomo_create(8, 8, 32, None) == -1
+    assert lib.lib.pt_tomo_destroy(None) == 0
+    with pytest.raises(lib.PtError):
+        lib.call("pt_k_norm2", None, None, 10, 1, None)
+
+
+def test_no_cpu_fallback_when_library_missing(tmp_path, monkeypatch):
+    """The package refuses to import its native binding without the library."""
+    import importlib
+    import sys
+    monkeypatch.setenv("PTYCHOTOMO_HIP_LIB", str(tmp_path / "missing.so"))
+    sys.modules.pop("ptychotomo_amd._lib", None)
+    with pytest.raises(ImportError):
+        importlib.import_module("ptychotomo_amd._lib")
+    monkeypatch.delenv("PTYCHOTOMO_HIP_LIB")
+    sys.modules.pop("ptychotomo_amd._lib", None)
+    importlib.import_module("ptychotomo_amd._lib")
