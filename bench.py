@@ -84,7 +84,7 @@ def cpu_baseline(w, budget_s=30.0):
     cores = len(os.sched_getaffinity(0))
     theta, prb, scan, obj = synthetic.make_inputs(w)
     rng = np.random.default_rng(0)
-    pz = 2
+    pz = 8
     t0 = time.perf_counter()
     ousfft.interp_matrix(theta, w.n)  # geometry set-up, not counted
     t_setup = time.perf_counter() - t0
@@ -95,7 +95,7 @@ def cpu_baseline(w, budget_s=30.0):
     t0 = time.perf_counter()
     ousfft.adj(g, theta, workers=cores)
     t_ta = (time.perf_counter() - t0) * (w.nz / pz)
-    na = 8
+    na = 32
     psi = np.exp(1j * 1e-2 * rng.standard_normal((na, w.nz, w.n))).astype(np.complex64)
     sc = scan[:, :na]
     t0 = time.perf_counter()
@@ -130,6 +130,10 @@ def main():
     ap.add_argument("--workload", default="prod256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="per-kernel time per iteration")
+    ap.add_argument("--no-breakdown-probe", action="store_true",
+                    help="skip the per-kernel probe iteration (short runs under counter collection)")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run 5 device copies of MIB MiB (HBM counter calibration)")
     ap.add_argument("--tol-run", type=int, default=0,
                     help="also run N plain iterations for time-to-Lagrangian-tolerance")
     args = ap.parse_args()
@@ -142,6 +146,17 @@ def main():
 
     import torch
     from ptychotomo_amd import _lib, synthetic
+
+    if args.copy_calib:
+        nbytes = args.copy_calib << 20
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes_read": nbytes, "copy_calib_bytes_written": nbytes,
+                          "copies": 5}), flush=True)
+        return
 
     w = synthetic.CONFIGS[args.workload]
     log = lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -163,13 +178,15 @@ def main():
     lib = _lib.lib
     saved = dict(slv.state)
     breakdown = {}
-    for tag in (TAGS if args.breakdown else ["tomo_gather", "tomo_scatter", "ptycho_grad_tile",
-                                              "ptycho_minf"]):
+    probe = TAGS if args.breakdown else ["tomo_gather", "tomo_scatter", "ptycho_grad_tile",
+                                         "ptycho_minf"]
+    for tag in ([] if args.no_breakdown_probe else probe):
         slv.state = dict(saved)
         ms, cnt = prof_run(lib, tag, lambda: slv.admm_resume(data, niter=1, lagr_every=10, **run))
         breakdown[tag] = {"ms_per_iter": ms, "launches": cnt}
     slv.state = dict(saved)
-    dom = max(breakdown, key=lambda k: breakdown[k]["ms_per_iter"])
+    dom = (max(breakdown, key=lambda k: breakdown[k]["ms_per_iter"]) if breakdown
+           else "tomo_scatter")
     log(f"breakdown {breakdown}; timing {args.steps} iterations, roofline kernel {dom}")
 
     # timed region
@@ -186,10 +203,12 @@ def main():
     nbytes = kernel_bytes(dom, slv)
     achieved = nbytes / (avg_ms * 1e-3) / 1e9 if nbytes else None
     traffic = None
-    tf = os.path.join(HERE, "profiles", f"{dom}_{args.workload}_traffic.json")
+    tf = os.path.join(HERE, "profiles", f"{dom}_{args.workload}_traffic.txt")
     if os.path.exists(tf):
         with open(tf) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            for line in fh:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    traffic = float(line.split(":", 1)[1])
 
     out = {
         "metric": "ADMM iters/s + time-to-Lagrangian-tol, 256^3, plain vs warm-start, 1 MI355X",
