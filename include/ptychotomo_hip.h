@@ -33,6 +33,10 @@ extern "C" {
 #define PT_MODEL_GAUSSIAN 0
 #define PT_MODEL_POISSON 1
 
+/* Detector-frequency filters of the tomography adjoint. */
+#define PT_FILTER_NONE 0
+#define PT_FILTER_RAMP 1
+
 typedef struct pt_tomo pt_tomo;
 typedef struct pt_ptycho pt_ptycho;
 typedef void* pt_stream; /* hipStream_t */
@@ -73,6 +77,11 @@ int pt_tomo_adj(pt_tomo* h, void* f, const void* g, pt_stream stream);
  * multiply by coeftomo after the native call). */
 int pt_tomo_fwd_scaled(pt_tomo* h, void* g, const void* f, float scale, pt_stream stream);
 int pt_tomo_adj_scaled(pt_tomo* h, void* f, const void* g, float scale, pt_stream stream);
+/* Filter applied by every later adjoint between its detector DFT and the
+ * gridding: PT_FILTER_RAMP is applyfilter (kernels.cuh:174-187, the
+ * commented-out step at radonusfft.cu:121), i.e. filtered backprojection;
+ * PT_FILTER_NONE (the default) restores radonusfft::adj.  Blocking. */
+int pt_tomo_set_filter(pt_tomo* h, int kind);
 
 /* ---------------------------------------------------------------- ptycho
  * Probe x object patch, D x D far field per scan position.

@@ -174,7 +174,18 @@ def fwd(u, theta, n=None, workers=None, dtype=np.complex64):
     return np.ascontiguousarray(g[:, :, o:o + n])
 
 
-def adj(g, theta, n=None, workers=None, dtype=np.complex64):
+def ramp_filter(N):
+    """applyfilter (kernels.cuh:174-187) weights per detector frequency index
+    i: rho = (i - N/2)/N, w = |rho| N 4 sin(rho)/rho, w = 0 at rho = 0, in
+    float32 as there."""
+    rho = ((np.arange(N) - N // 2) / np.float32(N)).astype(np.float32)
+    w = np.zeros(N, np.float32)
+    nz = rho != 0
+    w[nz] = (np.abs(rho[nz]) * np.float32(N) * np.float32(4) * np.sin(rho[nz]) / rho[nz]).astype(np.float32)
+    return w
+
+
+def adj(g, theta, n=None, workers=None, dtype=np.complex64, filt=None):
     """radonusfft::adj (radonusfft.cu:94-139), unnormalised.
 
     g: [ntheta, pz, n] complex64 -> [pz, n, n] complex64.  Note the detector
@@ -197,6 +208,8 @@ def adj(g, theta, n=None, workers=None, dtype=np.complex64):
     gg = gg * s1
     gg = sfft.fft(gg, axis=-1, workers=workers).astype(cdt)  # CUFFT_FORWARD
     gg = gg * s1
+    if filt == "ramp":                                       # applyfilter
+        gg = gg * ramp_filter(N)
     W = interp_matrix(theta, n, rdt)
     samples = gg.transpose(1, 0, 2).reshape(pz, ntheta * N)
     fdee = np.asarray((W.T @ samples.T).T, dtype=cdt).reshape(pz, G, G)  # scatter

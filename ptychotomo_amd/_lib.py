@@ -24,6 +24,7 @@ c_double = ctypes.c_double
 vp = ctypes.c_void_p
 PT_OK = 0
 PT_MODEL = {"gaussian": 0, "poisson": 1}
+PT_FILTER = {None: 0, "none": 0, "ramp": 1}
 
 _SIGS = {
     "pt_last_error": (ctypes.c_char_p, []),
@@ -37,6 +38,7 @@ _SIGS = {
     "pt_tomo_adj": (c_int, [vp, vp, vp, vp]),
     "pt_tomo_fwd_scaled": (c_int, [vp, vp, vp, c_float, vp]),
     "pt_tomo_adj_scaled": (c_int, [vp, vp, vp, c_float, vp]),
+    "pt_tomo_set_filter": (c_int, [vp, c_int]),
     "pt_ptycho_create": (c_int, [c_int] * 8 + [ctypes.POINTER(vp)]),
     "pt_ptycho_destroy": (c_int, [vp]),
     "pt_ptycho_set_scan_probe": (c_int, [vp, vp, vp, vp]),

@@ -134,8 +134,41 @@ __global__ __launch_bounds__(WAVE) void k_fwd(float2* __restrict__ g, const floa
 }
 
 // ------------------------------------------------------------------ minf
-// sums over patches of the data term at ngam candidates; one partial per
-// (chunk, block) in `partial[(c*gridDim.x + b)*ngam + j]`.
+// log|a| for |a|^2 = s, with |a| clamped below at 1e-32 as mlog does
+// (solver.py:48-52); accurate for every s (used where s < FLT_MIN).
+__device__ __forceinline__ float log_abs_exact(float2 a) {
+  return logf(fmaxf(sqrtf(a.x * a.x + a.y * a.y), 1e-32f));
+}
+
+// Poisson data term of one lane's far-field column at step gj: the hardware
+// log for |a|^2 >= FLT_MIN, the accurate path for the (rare) rest.
+template <int D>
+__device__ __forceinline__ float poisson_part(const float2 (&a)[D], const float2 (&b)[D],
+                                              const float (&d)[D], float gj) {
+  float part = 0.f;
+  bool rare = false;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const float2 ag = make_float2(a[k].x + gj * b[k].x, a[k].y + gj * b[k].y);
+    const float s2 = ag.x * ag.x + ag.y * ag.y;
+    rare |= !(s2 >= 1.17549435e-38f);
+    part += s2 - d[k] * __logf(s2);  // 2 d log|a| = d log|a|^2
+  }
+  if (rare) {
+    part = 0.f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const float2 ag = make_float2(a[k].x + gj * b[k].x, a[k].y + gj * b[k].y);
+      part += ag.x * ag.x + ag.y * ag.y - 2.f * d[k] * log_abs_exact(ag);
+    }
+  }
+  return part;
+}
+
+// Sums over patches of the data term at ngam step candidates; one partial
+// per (chunk, block) in partial[(c*gridDim.x + b)*ngam + j].  The candidate
+// loop is not unrolled and its FP64 accumulators rotate in registers; the
+// LDS footprint is the transpose buffer only.
 #define PT_MAX_GAM 8
 template <int D>
 __global__ __launch_bounds__(WAVE) void k_minf(double* __restrict__ partial,
@@ -150,12 +183,13 @@ __global__ __launch_bounds__(WAVE) void k_minf(double* __restrict__ partial,
                                                int lagr) {
   constexpr int G = WAVE / D;
   __shared__ float2 T[G * D * (D + 1)];
-  __shared__ double red[PT_MAX_GAM * 16];
   const int lane = lane_id(), slot = lane / D, r = lane % D;
   const int c = blockIdx.y;
-  float gam[PT_MAX_GAM];
-  for (int j = 0; j < PT_MAX_GAM; ++j) gam[j] = j < ngam ? (float)gammas[c * ngam + j] : 0.f;
+  __shared__ float gsh[PT_MAX_GAM];
+  if (lane < PT_MAX_GAM) gsh[lane] = lane < ngam ? (float)gammas[(long)c * ngam + lane] : 0.f;
+  __syncthreads();
   double acc[PT_MAX_GAM];
+#pragma unroll
   for (int j = 0; j < PT_MAX_GAM; ++j) acc[j] = 0.0;
   const long npatch = (long)chunk * S;
   const long base = (long)c * chunk * S;
@@ -166,6 +200,7 @@ __global__ __launch_bounds__(WAVE) void k_minf(double* __restrict__ partial,
     int2 ps = active ? pos[p] : make_int2(-1, -1);
     const int t = active ? (int)(p / S) : 0;
     float2 a[D], b[D];
+    float d[D];
     load_frame_row<D>(a, f, prb, ps, t, r, nz, n, P, ox, oy, cs, active);
     frame_fwd<D>(a, T, slot, r);
     if (dir) {
@@ -175,39 +210,52 @@ __global__ __launch_bounds__(WAVE) void k_minf(double* __restrict__ partial,
 #pragma unroll
       for (int k = 0; k < D; ++k) b[k] = make_float2(0.f, 0.f);
     }
-    if (active) {
-      const float* dp = data + p * D * D + r;
-      float part[PT_MAX_GAM];
-      for (int j = 0; j < PT_MAX_GAM; ++j) part[j] = 0.f;
+    if (!active) continue;
+    const float* dp = data + p * D * D + r;
+    float dterm = 0.f;
 #pragma unroll
-      for (int k = 0; k < D; ++k) {
-        const float d = dp[k * D] * cd;
-        const float2 A = cscale(a[k], cp), Bv = cscale(b[k], cp);
-        float dterm = 0.f;
-        if (lagr && model == PT_MODEL_POISSON) dterm = d - 2.f * d * logf(fmaxf(sqrtf(d), 1e-32f));
-        for (int j = 0; j < ngam; ++j) {
-          // fpsi + gamma*fd formed element-wise in FP32 (solver.py:172): a step
-          // below FP32 resolution leaves the term bit-identical to gamma = 0
-          const float gj = gam[j];
-          const float2 ag = make_float2(A.x + gj * Bv.x, A.y + gj * Bv.y);
-          const float s = ag.x * ag.x + ag.y * ag.y;
-          const float m = sqrtf(s);
-          float term;
-          if (model == PT_MODEL_POISSON) {
-            term = s - 2.f * d * logf(fmaxf(m, 1e-32f)) - dterm;
-          } else {
-            const float e = m - sqrtf(d);
-            term = e * e;
-          }
-          part[j] += term;
-        }
+    for (int k = 0; k < D; ++k) {
+      d[k] = dp[k * D] * cd;
+      a[k] = cscale(a[k], cp);
+      b[k] = cscale(b[k], cp);
+      if (model == PT_MODEL_GAUSSIAN) {
+        d[k] = sqrtf(d[k]);
+      } else if (lagr) {
+        dterm += d[k] - 2.f * d[k] * logf(fmaxf(sqrtf(d[k]), 1e-32f));
       }
-      for (int j = 0; j < ngam; ++j) acc[j] += (double)part[j];
+    }
+    // fpsi + gamma*fd formed element-wise in FP32 (solver.py:172): a step
+    // below FP32 resolution leaves the term bit-identical to gamma = 0.
+    // Candidate j accumulates into acc[0]; the accumulators rotate once per
+    // candidate slot (8 rotations per patch group restore their order).
+#pragma unroll 1
+    for (int j = 0; j < PT_MAX_GAM; ++j) {
+      if (j < ngam) {
+        const float gj = gsh[j];
+        float part;
+        if (model == PT_MODEL_POISSON) {
+          part = poisson_part<D>(a, b, d, gj) - dterm;
+        } else {
+          part = 0.f;
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            const float2 ag = make_float2(a[k].x + gj * b[k].x, a[k].y + gj * b[k].y);
+            const float e = sqrtf(ag.x * ag.x + ag.y * ag.y) - d[k];
+            part += e * e;
+          }
+        }
+        acc[0] += (double)part;
+      }
+      const double t0 = acc[0];
+#pragma unroll
+      for (int q = 0; q < PT_MAX_GAM - 1; ++q) acc[q] = acc[q + 1];
+      acc[PT_MAX_GAM - 1] = t0;
     }
   }
-  // block (single wave) reduction
-  for (int j = 0; j < ngam; ++j) {
-    double v = wave_sum(acc[j]);
+#pragma unroll
+  for (int j = 0; j < PT_MAX_GAM; ++j) {
+    if (j >= ngam) break;
+    const double v = wave_sum(acc[j]);
     if (lane == 0) partial[((long)c * gridDim.x + blockIdx.x) * ngam + j] = v;
   }
 }
@@ -414,7 +462,7 @@ int run_minf(pt_ptycho* h, double* sums, const float2* f, const float2* dir, con
   constexpr int G = WAVE / D;
   const int nchunk = h->pt / chunk;
   long np = (long)chunk * h->S;
-  int nb = launch_blocks(np, G, std::max(1, 4096 / nchunk));
+  int nb = launch_blocks(np, 4 * G, std::max(1, 20480 / nchunk));
   size_t need = (size_t)nchunk * nb * ngam;
   if (need > h->partial_cap) {
     if (h->partial) PT_HIP(hipFree(h->partial));

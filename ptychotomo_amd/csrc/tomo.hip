@@ -46,6 +46,7 @@ struct pt_tomo {
   float2* tw = nullptr;      // [N] exp(-2 pi i m/N)
   float2* twist = nullptr;   // [N] exp(-i pi j/N), j = idx - N/2
   float* dphi = nullptr;     // [N] exp(mu a^2)/(2N), a = idx - N/2
+  float* filt = nullptr;     // [N] detector-frequency filter of the adjoint, or null
   int2* mrange = nullptr;    // [N] circ mask column range [lo, hi) per row
   float2* spos = nullptr;    // [ntheta*N] sample positions (x, y)
   int2* sfloor = nullptr;    // [ntheta*N] floor(2N*x), floor(2N*y)
@@ -247,8 +248,8 @@ __global__ __launch_bounds__(256) void k_line_idft(float2* __restrict__ g, const
 // the owner-computes scatter.
 constexpr int kZQ = 8;
 __global__ __launch_bounds__(256) void k_adj_line(float2* __restrict__ Q, const float2* __restrict__ g,
-                           const float2* __restrict__ tw, LinePlan plan, float oscale, int n,
-                           int ntheta, int pz, int zq) {
+                           const float2* __restrict__ tw, const float* __restrict__ filt,
+                           LinePlan plan, float oscale, int n, int ntheta, int pz, int zq) {
   extern __shared__ float2 lds[];
   const int N = plan.N, th = blockIdx.x, z0 = blockIdx.y * zq;
   const int nzl = min(zq, pz - z0);
@@ -271,7 +272,8 @@ __global__ __launch_bounds__(256) void k_adj_line(float2* __restrict__ Q, const 
     if (zz >= nzl) continue;
     int b = i - N / 2;  // kappa
     if (b < 0) b += N;
-    Q[((long)th * N + i) * pz + z0 + zz] = cscale(lds[zz * N + b], oscale);
+    const float w = filt ? oscale * filt[i] : oscale;
+    Q[((long)th * N + i) * pz + z0 + zz] = cscale(lds[zz * N + b], w);
   }
 }
 
@@ -504,7 +506,7 @@ int run_adj(pt_tomo* h, float2* f, const float2* g, float scale, hipStream_t st)
     dim3 grid(h->ntheta, (h->pz + zq - 1) / zq);
     ProfScope prof("tomo_adj_line", st);
     hipLaunchKernelGGL(k_adj_line, grid, dim3(256), sizeof(float2) * zq * N, st, h->Q, g, h->tw,
-                       h->plan, oscale, h->n, h->ntheta, h->pz, zq);
+                       h->filt, h->plan, oscale, h->n, h->ntheta, h->pz, zq);
     PT_LAUNCH_CHECK();
   }
   {
@@ -682,7 +684,7 @@ static int upload(T** dst, const std::vector<T>& v, hipStream_t st) {
 
 extern "C" int pt_tomo_destroy(pt_tomo* h) {
   if (!h) return PT_OK;
-  (void)hipFree(h->tw); (void)hipFree(h->twist); (void)hipFree(h->dphi); (void)hipFree(h->mrange);
+  (void)hipFree(h->tw); (void)hipFree(h->twist); (void)hipFree(h->dphi); (void)hipFree(h->filt); (void)hipFree(h->mrange);
   (void)hipFree(h->spos); (void)hipFree(h->sfloor); (void)hipFree(h->A); (void)hipFree(h->Bg); (void)hipFree(h->Q);
   (void)hipFree(h->bin_sample);
   free_bins(h);
@@ -830,6 +832,28 @@ static int tomo_adj_impl(pt_tomo* h, void* f, const void* g, float scale, pt_str
   PT_REQUIRE(h && g && f, PT_ERR_ARG, "pt_tomo_adj: null argument");
   PT_REQUIRE(h->ready, PT_ERR_STATE, "pt_tomo_adj: set_theta not called");
   return run_adj(h, (float2*)f, (const float2*)g, scale, (hipStream_t)stream);
+}
+
+extern "C" int pt_tomo_set_filter(pt_tomo* h, int kind) {
+  PT_REQUIRE(h, PT_ERR_ARG, "pt_tomo_set_filter: null handle");
+  PT_REQUIRE(kind == PT_FILTER_NONE || kind == PT_FILTER_RAMP, PT_ERR_ARG,
+             "pt_tomo_set_filter: unknown filter kind");
+  if (kind == PT_FILTER_NONE) {
+    if (h->filt) PT_HIP(hipFree(h->filt));
+    h->filt = nullptr;
+    return PT_OK;
+  }
+  // applyfilter (kernels.cuh:174-187): w = |rho| N 4 sin(rho)/rho, rho =
+  // (i - N/2)/N, w = 0 at rho = 0; evaluated in FP32 as there
+  const int N = h->N;
+  std::vector<float> w(N);
+  for (int i = 0; i < N; ++i) {
+    const float rho = (float)(i - N / 2) / (float)N;
+    w[i] = rho == 0.f ? 0.f : std::fabs(rho) * (float)N * 4.f * std::sin(rho) / rho;
+  }
+  if (!h->filt) PT_HIP(hipMalloc(&h->filt, sizeof(float) * N));
+  PT_HIP(hipMemcpy(h->filt, w.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+  return PT_OK;
 }
 
 extern "C" int pt_tomo_fwd(pt_tomo* h, void* g, const void* f, pt_stream stream) {

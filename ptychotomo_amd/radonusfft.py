@@ -27,6 +27,11 @@ class radonusfft:
     def adj(self, f, g):
         _lib.call("pt_tomo_adj", self._h, ctypes.c_void_p(int(f)), ctypes.c_void_p(int(g)), stream())
 
+    def setfilter(self, kind):
+        """Filter of later adjoints: "ramp" (applyfilter, kernels.cuh:174-187,
+        i.e. filtered backprojection) or None (radonusfft::adj)."""
+        _lib.call("pt_tomo_set_filter", self._h, _lib.PT_FILTER[kind])
+
     # scaled variants used by Solver (coeftomo folded into the last kernel)
     def fwd_scaled(self, g, f, scale):
         _lib.call("pt_tomo_fwd_scaled", self._h, ctypes.c_void_p(int(g)), ctypes.c_void_p(int(f)),

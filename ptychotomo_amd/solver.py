@@ -180,6 +180,26 @@ class Solver(object):
         self._tomo.adj_scaled(res.data_ptr(), data.data_ptr(), float(self.coeftomo) * scale)
         return res
 
+    def fbp_tomo_batch(self, data):
+        """Filtered backprojection of projections scaled like fwd_tomo_batch:
+        the adjoint with the reference's ramp filter (applyfilter,
+        kernels.cuh:174-187; commented out at radonusfft.cu:121), normalised so
+        that fbp_tomo_batch(fwd_tomo_batch(u)) approximates u.  The filter
+        weight is about 4N|k|/N per detector frequency k, N = 3n/2, so the
+        filtered backprojection over ntheta angles in [0, pi) is
+        4 N ntheta / pi times the inverse Radon transform; coeftomo enters
+        once from the data and once from the adjoint."""
+        data = to_dev(data, c64)
+        res = empty((self.nz, self.n, self.n))
+        N = 3 * self.n // 2
+        scale = float(np.pi / (4.0 * N * self.ntheta * float(self.coeftomo)))
+        self._tomo.setfilter("ramp")
+        try:
+            self._tomo.adj_scaled(res.data_ptr(), data.data_ptr(), scale)
+        finally:
+            self._tomo.setfilter(None)
+        return res
+
     # ------------------------------------------------------------------ ptychography
     def fwd_ptycho(self, psi):
         """solver.py:100-105 (chunk set by cl_ptycho.setobj)."""

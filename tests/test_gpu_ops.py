@@ -146,3 +146,49 @@ def test_bitwise_reproducible():
         tomo.adj(f.data_ptr(), gg.data_ptr())
         outs.append(f.cpu().numpy())
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+def test_tomo_ramp_filter_adj():
+    """Filtered adjoint (applyfilter) against the restatement, then the filter
+    switched off again restores the plain adjoint."""
+    from ptychotomo_amd.radonusfft import radonusfft
+    n, pz, ntheta = 32, 2, 20
+    print(f"ramp filter n={n} pz={pz} ntheta={ntheta}", flush=True)
+    rng = np.random.default_rng(11)
+    theta = np.linspace(0, np.pi, ntheta, endpoint=False).astype(np.float32)
+    g = crandn(rng, (ntheta, pz, n))
+    op = radonusfft(ntheta, pz, n)
+    th, gd = dev(theta), dev(g)
+    op.setobj(th.data_ptr())
+    back = torch.zeros((pz, n, n), dtype=torch.complex64, device="cuda")
+    op.setfilter("ramp")
+    op.adj(back.data_ptr(), gd.data_ptr())
+    torch.cuda.synchronize()
+    ef = rel(back.cpu().numpy(), ousfft.adj(g, theta, filt="ramp"))
+    op.setfilter(None)
+    op.adj(back.data_ptr(), gd.data_ptr())
+    torch.cuda.synchronize()
+    ea = rel(back.cpu().numpy(), ousfft.adj(g, theta))
+    print(f"filtered adj rel {ef:.3e} plain adj rel {ea:.3e}")
+    assert ef < 1e-5, ef
+    assert ea < 1e-5, ea
+
+
+def test_solver_fbp_inverts_radon():
+    """Solver.fbp_tomo_batch(fwd_tomo_batch(u)) approximates u (inverse
+    Radon transform; error of the reference's ramp filter itself ~9 %)."""
+    from ptychotomo_amd import Solver, objects
+    n, nz, ntheta = 64, 4, 96
+    theta = np.linspace(0, np.pi, ntheta, endpoint=False).astype(np.float32)
+    prb = objects.gaussian_probe(16, 4.0, 0.5)
+    scan = np.zeros((2, ntheta, 4), np.float32)
+    slv = Solver(prb, scan, theta, 16, 1e-6, 8.8, ntheta, nz, n, ntheta, nz)
+    y, x = np.mgrid[:n, :n] - n / 2 + 0.5
+    u = np.exp(-(x ** 2 + y ** 2) / (2 * (n / 10) ** 2)).astype(np.complex64)
+    u = np.repeat(u[None], nz, 0) * (1 + 0.1j)
+    rec = slv.fbp_tomo_batch(slv.fwd_tomo_batch(u)).cpu().numpy()
+    c = np.vdot(u, rec).real / np.vdot(u, u).real
+    err = rel(rec / c, u)
+    print(f"fbp scale {c:.4f} rel err {err:.3e}")
+    assert abs(c - 1) < 0.05, c
+    assert err < 0.12, err

@@ -266,3 +266,33 @@ def test_lagrangian_sum_omits_tau_term():
                        0.5, 0.5, 'poisson')
     assert abs(lagr[6] - np.sum(lagr[0:5], dtype=np.float64)) <= 1e-5 * abs(lagr[6])
     assert lagr[5] > 0
+
+
+def test_usfft_ramp_filter_inverts_radon():
+    """applyfilter's weight ~ 4N|k|/N makes the filtered adjoint a filtered
+    backprojection: FBP(R u) = (4 N ntheta / pi) u up to the filter's own
+    error (sin(rho)/rho taper, removed DC) for a smooth object.  The
+    unfiltered adjoint is a blurred u (error > 0.5), so a filter applied in
+    the wrong domain or with a wrong weight fails one of the two bounds."""
+    n, ntheta = 64, 96
+    N = 3 * n // 2
+    theta = np.linspace(0, np.pi, ntheta, endpoint=False).astype(np.float32)
+    y, x = np.mgrid[:n, :n] - n / 2 + 0.5
+    u = np.exp(-(x ** 2 + y ** 2) / (2 * (n / 10) ** 2)).astype(np.complex64)[None]
+    g = ousfft.fwd(u, theta)
+    for filt, lo, hi in (("ramp", 0.0, 0.12), (None, 0.5, 10.0)):
+        f = ousfft.adj(g, theta, filt=filt)
+        c = np.vdot(u, f).real / np.vdot(u, u).real
+        err = np.linalg.norm(f / c - u) / np.linalg.norm(u)
+        assert lo <= err < hi, (filt, err)
+        if filt:
+            assert abs(c / (4 * N * ntheta / np.pi) - 1) < 0.05, c
+
+
+def test_usfft_ramp_filter_weights():
+    w = ousfft.ramp_filter(12)
+    assert w[6] == 0                                   # rho = 0
+    assert np.allclose(w, w[::-1][np.r_[11, 0:11]])    # even in rho about i = N/2
+    rho = (np.arange(12) - 6) / 12
+    m = rho != 0
+    assert np.allclose(w[m], 4 * 12 * np.abs(np.sin(rho[m])), rtol=1e-6)
