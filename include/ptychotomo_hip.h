@@ -116,10 +116,11 @@ int pt_ptycho_adj_scaled(pt_ptycho* h, void* f, const void* g, float scale, pt_s
  *        = fpsi - sqrt(d)*exp(i*angle(fpsi))        (gaussian)
  *   grad = coefptycho * F*(r) - rho*(h - psi + lamd/rho)
  * with d = coefdata*data, data: [ptheta, nscan, detx, dety] float32.
- * The far field is never written to memory. */
+ * The far field is never written to memory.  rho_dev, if not NULL, is a
+ * device FP64 scalar used instead of rho. */
 int pt_ptycho_grad(pt_ptycho* h, void* grad, const void* psi, const float* data, const void* hh,
-                   const void* lamd, double rho, float coefptycho, float coefdata, int model,
-                   pt_stream stream);
+                   const void* lamd, double rho, const double* rho_dev, float coefptycho,
+                   float coefdata, int model, pt_stream stream);
 /* Data term of the cg_ptycho functional (solver.py:208-212) at ngam step
  * candidates psi + gamma*d, segmented by angle chunks of `chunk` angles:
  *   sums[c*ngam + j] = sum_{chunk c} |a|^2 - 2 d mlog|a|   (poisson)
@@ -127,16 +128,21 @@ int pt_ptycho_grad(pt_ptycho* h, void* grad, const void* psi, const float* data,
  *   a = coefptycho*F(psi + gammas[c*ngam+j]*dir).
  * sums, gammas: device FP64 arrays.  dir may be NULL (then only gamma = 0
  * is meaningful).  With lagr != 0 the poisson sum also subtracts
- * d - 2 d mlog(sqrt d), giving lagr[0] of take_lagr (solver.py:294-296). */
+ * d - 2 d mlog(sqrt d), giving lagr[0] of take_lagr (solver.py:294-296).
+ * skip, if not NULL, is a line-search state (pt_k_ls_begin): chunks whose
+ * search has finished are not evaluated. */
 int pt_ptycho_minf(pt_ptycho* h, double* sums, const void* psi, const void* dir, const float* data,
                    const double* gammas, int ngam, int chunk, float coefptycho, float coefdata,
-                   int model, int lagr, pt_stream stream);
+                   int model, int lagr, const double* skip, pt_stream stream);
 
 /* ------------------------------------------------------- solver kernels
  * Complex64 vectors of length `len`, optionally split into `nseg` equal
  * contiguous segments (per-chunk scalars).  Reductions accumulate in FP64
  * and write their results to device memory.  Scalar arrays named *_dev are
  * device FP64 arrays indexed by segment (complex scalars as re,im pairs).
+ * A real scalar argument x with a companion `const double* x_dev` is read
+ * from device memory when x_dev is not NULL (the device-driven solver keeps
+ * rho, tau and the takexi phase there).
  */
 /* out[2*s..] = sum_seg conj(x)*y */
 int pt_k_dot(double* out, const void* x, const void* y, long len, int nseg, pt_stream stream);
@@ -165,33 +171,42 @@ int pt_k_reg_fwd(void* out, const void* u, int nz, int ny, int nx, pt_stream str
 int pt_k_reg_adj(void* out, const void* g, int nz, int ny, int nx, pt_stream stream);
 /* solve_reg (solver.py:257-264): z = J u + mu/tau, isotropic soft threshold at
  * alpha/tau over the 3 components. */
-int pt_k_reg_solve(void* out, const void* u, const void* mu, double tau, double alpha, int nz,
-                   int ny, int nx, pt_stream stream);
+int pt_k_reg_solve(void* out, const void* u, const void* mu, double tau, const double* tau_dev,
+                   double alpha, int nz, int ny, int nx, pt_stream stream);
 
 /* exptomo (solver.py:59-60) with an optional extra phase (solver.py:322):
- * out = exp(i*c*psi) * exp(i*phase). */
-int pt_k_exptomo(void* out, const void* psi, double c, double phase, long len, pt_stream stream);
+ * out = exp(i*c*psi) * exp(i*phase).  phase_dev, if not NULL, points to
+ * {phase, cos phase, sin phase} (the prm of pt_k_takexi_dev). */
+int pt_k_exptomo(void* out, const void* psi, double c, double phase, const double* phase_dev,
+                 long len, pt_stream stream);
 /* mlog (solver.py:48-52): log(x) with |x| < 1e-32 replaced by 1e-32. */
 int pt_k_mlog(void* out, const void* x, long len, pt_stream stream);
 
 /* takexi statistics (solver.py:151-156,164): out[0] = sum angle(psi[:,:,r:2r]),
  * out[1] = sum angle(psi[:,:,n-2r:n-r]), out[2] = max |psi - lamd/rho|. */
-int pt_k_takexi_stats(double* out, const void* psi, const void* lamd, double rho, int ntheta,
-                      int nz, int n, int r, pt_stream stream);
+int pt_k_takexi_stats(double* out, const void* psi, const void* lamd, double rho,
+                      const double* rho_dev, int ntheta, int nz, int n, int r, pt_stream stream);
 /* takexi body (solver.py:158-167): given pshift and kscale = c/max|K_raw|:
  *   t = (psi - lamd/rho) e^{-i pshift}; K = i*kc*t*kscale;
  *   xi0 = K * (-i mlog(t) / c1) * coeftomo. */
 int pt_k_takexi(void* xi0, void* K, const void* psi, const void* lamd, double rho, double pshift,
                 double kc, double kscale, double c1, double coeftomo, long len, pt_stream stream);
+/* takexi with its scalars formed on the device from pt_k_takexi_stats' out
+ * (stats) as solver.py:149-168 forms them: pshift = float32 mean of the two
+ * band means (cnt samples each), kscale = 1/(kc*max|t|) (0 if max|t| = 0);
+ * prm receives {pshift, cos pshift, sin pshift, kc*kscale}. */
+int pt_k_takexi_dev(void* xi0, void* K, double* prm, const void* psi, const void* lamd,
+                    const double* rho_dev, const double* stats, double cnt, double kc, double c1,
+                    double coeftomo, long len, pt_stream stream);
 
 /* cg_tomo pieces (solver.py:180-203), single segment:
- * r = conj(K)*(K*Ru - xi0), out[0] = ||K*Ru - xi0||^2 */
-int pt_k_tomo_resid(void* r, double* out, const void* K, const void* Ru, const void* xi0, long len,
-                    pt_stream stream);
+ * r = scale*conj(K)*(K*Ru - xi0), out[0] = ||K*Ru - xi0||^2 */
+int pt_k_tomo_resid(void* r, double* out, const void* K, const void* Ru, const void* xi0,
+                    double scale, const double* scale_dev, long len, pt_stream stream);
 /* grad += tau * J*(J u - xi1) (adj_reg of fwd_reg residual, solver.py:188-190),
  * out[0] = ||J u - xi1||^2.  u: [nz,ny,nx], xi1: [3,nz,ny,nx]. */
-int pt_k_reg_grad(void* grad, double* out, const void* u, const void* xi1, double tau, int nz,
-                  int ny, int nx, pt_stream stream);
+int pt_k_reg_grad(void* grad, double* out, const void* u, const void* xi1, double tau,
+                  const double* tau_dev, int nz, int ny, int nx, pt_stream stream);
 /* Line-search candidates (solver.py:171-177), evaluated as the reference
  * forms them: the stepped quantity x + gamma*dx element-wise in FP32, sums in
  * FP64, so a step below FP32 resolution reproduces the gamma = 0 value
@@ -199,14 +214,18 @@ int pt_k_reg_grad(void* grad, double* out, const void* u, const void* xi1, doubl
  * cg_ptycho rho term (solver.py:213), per angle-chunk segment:
  *   out[s*ngam+j] = ||h - (psi + g_j d) + lamd/rho||^2 */
 int pt_k_rho_minf(double* out, const void* h, const void* psi, const void* lamd, const void* d,
-                  double rho, const double* gammas, int ngam, long len, int nseg,
-                  pt_stream stream);
+                  double rho, const double* rho_dev, const double* gammas, int ngam, long len,
+                  int nseg, const double* skip, pt_stream stream);
 /* cg_tomo (solver.py:182-183): out[j] = ||K*Ru + g_j*(K*Rd) - xi0||^2 */
 int pt_k_tomo_minf(double* out, const void* K, const void* Ru, const void* Rd, const void* xi0,
-                   const double* gammas, int ngam, long len, pt_stream stream);
+                   const double* gammas, int ngam, long len, const double* skip,
+                   pt_stream stream);
 /* cg_tomo: out[j] = ||J u + g_j*J d - xi1||^2 */
 int pt_k_reg_minf(double* out, const void* u, const void* d, const void* xi1,
-                  const double* gammas, int ngam, int nz, int ny, int nx, pt_stream stream);
+                  const double* gammas, int ngam, int nz, int ny, int nx, const double* skip,
+                  pt_stream stream);
+/* (skip of the three: as in pt_ptycho_minf, segments whose line search has
+ * finished are not evaluated.) */
 /* out = sum conj(a)*(b - c) (complex, re/im) -- take_lagr terms 2 and 4 */
 int pt_k_dot_diff(double* out, const void* a, const void* b, const void* c, long len,
                   pt_stream stream);
@@ -222,6 +241,33 @@ int pt_k_axpy_seg(void* y, const double* a_dev, const void* x, long len, int nse
 int pt_k_intensity(float* out, const void* g, float scale, long len, pt_stream stream);
 /* out[0] = max |angle(x)| (phase-wrap check of solver.py:238-240) */
 int pt_k_max_angle(double* out, const void* x, long len, pt_stream stream);
+/* ADMM dual update (solver.py:326-327): out = lam + k*(a - b) with k = c,
+ * or k = 1/c if inv != 0; a may be NULL (0), giving xi1 = phi - mu/tau of
+ * takexi (solver.py:168).  out may alias lam. */
+int pt_k_dual(void* out, const void* lam, const void* a, const void* b, double c,
+              const double* c_dev, int inv, long len, pt_stream stream);
+
+/* ------------------------------------------------------- device-side decisions
+ * The scalar rules of the CG and ADMM loops evaluated on the device (FP64),
+ * so that an ADMM iteration enqueues without host round trips. */
+/* Dai-Yuan beta (solver.py:195-196, 230-231) from pt_k_dy_reduce's red
+ * [nseg][3]: beta[s] = ||g||^2 / conj-dot (re, im); 0 if the denominator is
+ * 0 or not finite (after a zero step, where the reference divides 0 by 0). */
+int pt_k_dai_yuan(double* beta, const double* red, int nseg, pt_stream stream);
+/* Halving line search (solver.py:171-177) per segment as sweeps of ngam
+ * candidates.  state [nseg][2] = {current step, done}: begin writes
+ * cand[s][0] = 0, cand[s][j] = cur*2^-(j-1) and clears done.  After the
+ * functional f = w[0]*f1 + w[1]*f2 (f2 may be NULL; w: 2 device FP64) is
+ * evaluated at cand, select accepts per segment the first candidate that is
+ * not a strict ascent or is <= 1e-12 (then 0), writing step[s] = {gamma, 0}
+ * and state = {gamma, 1}; unfinished segments get the next sweep's
+ * candidates; with last != 0 they end at 0. */
+int pt_k_ls_begin(double* state, double* cand, int nseg, int ngam, pt_stream stream);
+int pt_k_ls_select(double* state, double* cand, double* step, const double* f1, const double* f2,
+                   const double* w, int nseg, int ngam, int last, pt_stream stream);
+/* Residual balancing (solver.py:267-282): rho_tau = {rho, tau} updated in
+ * place from norms = {||psi-h||^2, ||h-h0||^2, ||phi-e||^2, ||e-e0||^2}. */
+int pt_k_penalty(double* rho_tau, const double* norms, pt_stream stream);
 
 #ifdef __cplusplus
 }

@@ -46,9 +46,10 @@ _SIGS = {
     "pt_ptycho_adj": (c_int, [vp, vp, vp, vp]),
     "pt_ptycho_fwd_scaled": (c_int, [vp, vp, vp, c_float, vp]),
     "pt_ptycho_adj_scaled": (c_int, [vp, vp, vp, c_float, vp]),
-    "pt_ptycho_grad": (c_int, [vp, vp, vp, vp, vp, vp, c_double, c_float, c_float, c_int, vp]),
+    "pt_ptycho_grad": (c_int, [vp, vp, vp, vp, vp, vp, c_double, vp, c_float, c_float, c_int,
+                               vp]),
     "pt_ptycho_minf": (c_int, [vp, vp, vp, vp, vp, vp, c_int, c_int, c_float, c_float, c_int,
-                               c_int, vp]),
+                               c_int, vp, vp]),
     "pt_k_dot": (c_int, [vp, vp, vp, c_long, c_int, vp]),
     "pt_k_norm2": (c_int, [vp, vp, c_long, c_int, vp]),
     "pt_k_norm2_lin": (c_int, [vp, vp, vp, c_double, c_double, c_double, c_double, c_long, c_int,
@@ -58,17 +59,20 @@ _SIGS = {
     "pt_k_dy_reduce": (c_int, [vp, vp, vp, vp, c_long, c_int, vp]),
     "pt_k_reg_fwd": (c_int, [vp, vp, c_int, c_int, c_int, vp]),
     "pt_k_reg_adj": (c_int, [vp, vp, c_int, c_int, c_int, vp]),
-    "pt_k_reg_solve": (c_int, [vp, vp, vp, c_double, c_double, c_int, c_int, c_int, vp]),
-    "pt_k_exptomo": (c_int, [vp, vp, c_double, c_double, c_long, vp]),
+    "pt_k_reg_solve": (c_int, [vp, vp, vp, c_double, vp, c_double, c_int, c_int, c_int, vp]),
+    "pt_k_exptomo": (c_int, [vp, vp, c_double, c_double, vp, c_long, vp]),
     "pt_k_mlog": (c_int, [vp, vp, c_long, vp]),
-    "pt_k_takexi_stats": (c_int, [vp, vp, vp, c_double, c_int, c_int, c_int, c_int, vp]),
+    "pt_k_takexi_stats": (c_int, [vp, vp, vp, c_double, vp, c_int, c_int, c_int, c_int, vp]),
     "pt_k_takexi": (c_int, [vp, vp, vp, vp, c_double, c_double, c_double, c_double, c_double,
                             c_double, c_long, vp]),
-    "pt_k_tomo_resid": (c_int, [vp, vp, vp, vp, vp, c_long, vp]),
-    "pt_k_reg_grad": (c_int, [vp, vp, vp, vp, c_double, c_int, c_int, c_int, vp]),
-    "pt_k_rho_minf": (c_int, [vp, vp, vp, vp, vp, c_double, vp, c_int, c_long, c_int, vp]),
-    "pt_k_tomo_minf": (c_int, [vp, vp, vp, vp, vp, vp, c_int, c_long, vp]),
-    "pt_k_reg_minf": (c_int, [vp, vp, vp, vp, vp, c_int, c_int, c_int, c_int, vp]),
+    "pt_k_takexi_dev": (c_int, [vp, vp, vp, vp, vp, vp, vp, c_double, c_double, c_double,
+                                c_double, c_long, vp]),
+    "pt_k_tomo_resid": (c_int, [vp, vp, vp, vp, vp, c_double, vp, c_long, vp]),
+    "pt_k_reg_grad": (c_int, [vp, vp, vp, vp, c_double, vp, c_int, c_int, c_int, vp]),
+    "pt_k_rho_minf": (c_int, [vp, vp, vp, vp, vp, c_double, vp, vp, c_int, c_long, c_int, vp,
+                              vp]),
+    "pt_k_tomo_minf": (c_int, [vp, vp, vp, vp, vp, vp, c_int, c_long, vp, vp]),
+    "pt_k_reg_minf": (c_int, [vp, vp, vp, vp, vp, c_int, c_int, c_int, c_int, vp, vp]),
     "pt_k_dot_diff": (c_int, [vp, vp, vp, vp, c_long, vp]),
     "pt_k_tv_norm": (c_int, [vp, vp, c_long, vp]),
     "pt_k_lincomb3": (c_int, [vp, c_double, c_double, vp, c_double, c_double, vp, c_double,
@@ -76,6 +80,11 @@ _SIGS = {
     "pt_k_axpy_seg": (c_int, [vp, vp, vp, c_long, c_int, vp]),
     "pt_k_max_angle": (c_int, [vp, vp, c_long, vp]),
     "pt_k_intensity": (c_int, [vp, vp, c_float, c_long, vp]),
+    "pt_k_dual": (c_int, [vp, vp, vp, vp, c_double, vp, c_int, c_long, vp]),
+    "pt_k_dai_yuan": (c_int, [vp, vp, c_int, vp]),
+    "pt_k_ls_begin": (c_int, [vp, vp, c_int, c_int, vp]),
+    "pt_k_ls_select": (c_int, [vp, vp, vp, vp, vp, vp, c_int, c_int, c_int, vp]),
+    "pt_k_penalty": (c_int, [vp, vp, vp]),
 }
 
 EXPORTED = sorted(_SIGS)

@@ -82,6 +82,50 @@ __device__ __forceinline__ void frame_inv(float2 (&v)[D], float2* T, int slot, i
   reg_fft<D, 1>(v);
 }
 
+// Wave-local LDS exchange: orders the wave's LDS accesses (LDS operations of
+// one wavefront complete in issue order) without a workgroup barrier.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// D x D transposes through a per-wave float buffer (real, then imaginary
+// parts: half the LDS of a float2 buffer).  Row stride D+1; the slot stride
+// D(D+1) puts the two slots of a half-wave on disjoint banks.
+template <int D, bool ROW_TO_COL>
+__device__ __forceinline__ void transpose_w(float2 (&v)[D], float* Ts, int r) {
+  float tmp[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) Ts[ROW_TO_COL ? r * (D + 1) + k : k * (D + 1) + r] = v[k].x;
+  wave_sync();
+#pragma unroll
+  for (int k = 0; k < D; ++k) tmp[k] = Ts[ROW_TO_COL ? k * (D + 1) + r : r * (D + 1) + k];
+  wave_sync();
+#pragma unroll
+  for (int k = 0; k < D; ++k) Ts[ROW_TO_COL ? r * (D + 1) + k : k * (D + 1) + r] = v[k].y;
+  wave_sync();
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    v[k] = make_float2(tmp[k], Ts[ROW_TO_COL ? k * (D + 1) + r : r * (D + 1) + k]);
+  }
+  wave_sync();
+}
+// frame_fwd / frame_inv with the wave-local float buffer (T: this wave's
+// G*D*(D+1) floats)
+template <int D>
+__device__ __forceinline__ void frame_fwd_w(float2 (&v)[D], float* T, int slot, int r) {
+  reg_fft<D, -1>(v);
+  transpose_w<D, true>(v, T + slot * D * (D + 1), r);
+  reg_fft<D, -1>(v);
+}
+template <int D>
+__device__ __forceinline__ void frame_inv_w(float2 (&v)[D], float* T, int slot, int r) {
+  reg_fft<D, 1>(v);
+  transpose_w<D, false>(v, T + slot * D * (D + 1), r);
+  reg_fft<D, 1>(v);
+}
+
 // Load frame row r of patch p: c * prb[ix][iy] * psi[t][sty+iy][stx+ix]
 template <int D>
 __device__ __forceinline__ void load_frame_row(float2 (&v)[D], const float2* __restrict__ f,
@@ -180,11 +224,13 @@ __global__ __launch_bounds__(WAVE) void k_minf(double* __restrict__ partial,
                                                const double* __restrict__ gammas, int ngam,
                                                int chunk, int nz, int n, int S, int P, int ox,
                                                int oy, float cs, float cp, float cd, int model,
-                                               int lagr) {
+                                               int lagr, const double* __restrict__ skip) {
   constexpr int G = WAVE / D;
   __shared__ float2 T[G * D * (D + 1)];
   const int lane = lane_id(), slot = lane / D, r = lane % D;
   const int c = blockIdx.y;
+  // chunks whose line search has finished (skip[2*c+1] != 0) are not evaluated
+  if (skip && skip[2 * c + 1] != 0.0) return;
   __shared__ float gsh[PT_MAX_GAM];
   if (lane < PT_MAX_GAM) gsh[lane] = lane < ngam ? (float)gammas[(long)c * ngam + lane] : 0.f;
   __syncthreads();
@@ -261,11 +307,12 @@ __global__ __launch_bounds__(WAVE) void k_minf(double* __restrict__ partial,
 }
 
 __global__ void k_reduce_partials(double* __restrict__ out, const double* __restrict__ partial,
-                                  int nblk, int ngam) {
+                                  int nblk, int ngam, const double* __restrict__ skip) {
   // one thread per (chunk, gamma): fixed-order sum over blocks
   const int c = blockIdx.x;
   const int j = threadIdx.x;
   if (j >= ngam) return;
+  if (skip && skip[2 * c + 1] != 0.0) return;
   double s = 0.0;
   for (int b = 0; b < nblk; ++b) s += partial[((long)c * nblk + b) * ngam + j];
   out[c * ngam + j] = s;
@@ -274,8 +321,13 @@ __global__ void k_reduce_partials(double* __restrict__ out, const double* __rest
 // ------------------------------------------------------------------ tiles
 enum { TILE_ADJ = 0, TILE_GRAD = 1 };
 
+// Waves per overlap-add workgroup: the waves share one LDS tile and add into
+// it in wave order (deterministic); small frames allow more waves.
+template <int D>
+constexpr int tile_waves() { return D <= 16 ? 4 : 2; }
+
 template <int D, int MODE>
-__global__ __launch_bounds__(WAVE) void k_tile(
+__global__ __launch_bounds__(WAVE * tile_waves<D>()) void k_tile(
     float2* __restrict__ scratch, const float2* __restrict__ gin, const float2* __restrict__ f,
     const float* __restrict__ data, const float2* __restrict__ prb, const int2* __restrict__ pos,
     const float2* __restrict__ shift, const int* __restrict__ tile_start,
@@ -283,9 +335,11 @@ __global__ __launch_bounds__(WAVE) void k_tile(
     int B, int EB, int nz, int n, int S, int P, int ox, int oy, float cs, float cp, float cd,
     float out_scale, int model) {
   constexpr int G = WAVE / D;
-  extern __shared__ float2 lds[];
-  float2* T = lds;                          // G*D*(D+1)
-  float2* acc = lds + G * D * (D + 1);      // EB*EB
+  constexpr int W = tile_waves<D>();
+  extern __shared__ float lds_f[];
+  float2* acc = reinterpret_cast<float2*>(lds_f);          // EB*EB
+  const int wave = threadIdx.x / WAVE;
+  float* T = lds_f + 2 * EB * EB + wave * G * D * (D + 1);  // this wave's buffer
   const long tile = blockIdx.x;
   const int slotid = tile_slot[tile];
   if (slotid < 0) return;
@@ -294,12 +348,12 @@ __global__ __launch_bounds__(WAVE) void k_tile(
   const int t = (int)(tile / tpa);
   const int rem = (int)(tile - (long)t * tpa);
   const int y0 = (rem / ntx) * B, x0 = (rem % ntx) * B;
-  for (int i = lane; i < EB * EB; i += WAVE) acc[i] = make_float2(0.f, 0.f);
+  for (int i = threadIdx.x; i < EB * EB; i += W * WAVE) acc[i] = make_float2(0.f, 0.f);
   __syncthreads();
   const int beg = tile_start[tile], end = tile_start[tile + 1];
   const int ix = r - ox;
-  for (int i0 = beg; i0 < end; i0 += G) {
-    const int ii = i0 + slot;
+  for (int i0 = beg; i0 < end; i0 += W * G) {
+    const int ii = i0 + wave * G + slot;
     const bool active = ii < end;
     const long p = active ? tile_patch[ii] : 0;
     const int2 ps = active ? pos[p] : make_int2(-1, -1);
@@ -317,7 +371,7 @@ __global__ __launch_bounds__(WAVE) void k_tile(
       }
     } else {
       load_frame_row<D>(v, f, prb, ps, t, r, nz, n, P, ox, oy, cs, active);
-      frame_fwd<D>(v, T, slot, r);
+      frame_fwd_w<D>(v, T, slot, r);
       // residual in the unshifted frame (the per-patch unimodular factor of
       // kernels.cuh:319-355 cancels: r(s*a) = s*r(a), F* removes s again)
       if (active) {
@@ -347,34 +401,46 @@ __global__ __launch_bounds__(WAVE) void k_tile(
         }
       }
     }
-    frame_inv<D>(v, T, slot, r);
-    // overlap-add of c*conj(prb)*frame into the LDS tile
-    if (active && ps.x >= 0 && ix >= 0 && ix < P) {
-      const float2* pp = prb + (long)ix * P - oy;
-      const int col = ps.x + ix - x0;
-      const int rowb = ps.y - oy - y0;
+    frame_inv_w<D>(v, T, slot, r);
+    // overlap-add of c*conj(prb)*frame into the LDS tile, one wave at a time
+    // in wave order (fixed summation order)
+    const bool adds = active && ps.x >= 0 && ix >= 0 && ix < P;
+    const float2* pp = prb + (long)ix * P - oy;
+    const int col = ps.x + ix - x0;
+    const int rowb = ps.y - oy - y0;
+#pragma unroll 1
+    for (int w = 0; w < W; ++w) {
+      if (w == wave && adds) {
 #pragma unroll
-      for (int k = 0; k < D; ++k) {
-        const int iy = k - oy;
-        if (iy >= 0 && iy < P) {
-          const float2 val = cscale(cmulc(v[k], pp[k]), out_scale);
-          float2* dst = acc + (rowb + k) * EB + col;
-          atomicAdd(&dst->x, val.x);
-          atomicAdd(&dst->y, val.y);
+        for (int k = 0; k < D; ++k) {
+          const int iy = k - oy;
+          if (iy >= 0 && iy < P) {
+            const float2 val = cscale(cmulc(v[k], pp[k]), out_scale);
+            float2* dst = acc + (rowb + k) * EB + col;
+            atomicAdd(&dst->x, val.x);
+            atomicAdd(&dst->y, val.y);
+          }
         }
       }
+      __syncthreads();
     }
   }
-  __syncthreads();
   float2* out = scratch + (long)slotid * EB * EB;
-  for (int i = lane; i < EB * EB; i += WAVE) out[i] = acc[i];
+  for (int i = threadIdx.x; i < EB * EB; i += W * WAVE) out[i] = acc[i];
+}
+
+// LDS bytes of a k_tile workgroup
+inline size_t tile_lds_bytes(int D, int EB) {
+  const int W = D <= 16 ? 4 : 2, G = 64 / D;
+  return sizeof(float2) * (size_t)EB * EB + sizeof(float) * (size_t)W * G * D * (D + 1);
 }
 
 template <int MODE>
 __global__ void k_fold(float2* __restrict__ out, const float2* __restrict__ scratch,
                        const int* __restrict__ tile_slot, const float2* __restrict__ psi,
-                       const float2* __restrict__ hh, const float2* __restrict__ lamd, float rho,
-                       int pt, int nz, int n, int nty, int ntx, int B, int EB) {
+                       const float2* __restrict__ hh, const float2* __restrict__ lamd, float rhov,
+                       const double* __restrict__ rho_dev, int pt, int nz, int n, int nty, int ntx,
+                       int B, int EB) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long total = (long)pt * nz * n;
   if (idx >= total) return;
@@ -399,6 +465,7 @@ __global__ void k_fold(float2* __restrict__ out, const float2* __restrict__ scra
   }
   if constexpr (MODE == TILE_GRAD) {
     // grad -= rho*(h - psi + lamd/rho)   (solver.py:225)
+    const float rho = rho_dev ? (float)rho_dev[0] : rhov;
     const float2 h = hh[idx], p = psi[idx], l = lamd[idx];
     const float ir = 1.0f / rho;
     const float2 q = make_float2(h.x - p.x + l.x * ir, h.y - p.y + l.y * ir);
@@ -444,10 +511,10 @@ int run_fwd(pt_ptycho* h, float2* g, const float2* f, float scale, hipStream_t s
 template <int D, int MODE>
 int run_tiles(pt_ptycho* h, const float2* gin, const float2* f, const float* data, float cp,
               float cd, float out_scale, int model, hipStream_t st) {
-  constexpr int G = WAVE / D;
-  size_t lds = sizeof(float2) * ((size_t)G * D * (D + 1) + (size_t)h->EB * h->EB);
+  const size_t lds = tile_lds_bytes(D, h->EB);
   ProfScope prof(MODE == TILE_GRAD ? "ptycho_grad_tile" : "ptycho_adj_tile", st);
-  hipLaunchKernelGGL((k_tile<D, MODE>), dim3((unsigned)h->ntiles), dim3(WAVE), lds, st, h->scratch,
+  hipLaunchKernelGGL((k_tile<D, MODE>), dim3((unsigned)h->ntiles), dim3(WAVE * tile_waves<D>()),
+                     lds, st, h->scratch,
                      gin, f, data, h->prb, h->pos, h->shift, h->tile_start, h->tile_patch,
                      h->tile_slot, h->nty, h->ntx, h->B, h->EB, h->nz, h->n, h->S, h->P, h->ox,
                      h->oy, h->c, cp, cd, out_scale, model);
@@ -458,7 +525,7 @@ int run_tiles(pt_ptycho* h, const float2* gin, const float2* f, const float* dat
 template <int D>
 int run_minf(pt_ptycho* h, double* sums, const float2* f, const float2* dir, const float* data,
              const double* gammas, int ngam, int chunk, float cp, float cd, int model, int lagr,
-             hipStream_t st) {
+             const double* skip, hipStream_t st) {
   constexpr int G = WAVE / D;
   const int nchunk = h->pt / chunk;
   long np = (long)chunk * h->S;
@@ -474,22 +541,23 @@ int run_minf(pt_ptycho* h, double* sums, const float2* f, const float2* dir, con
   ProfScope prof("ptycho_minf", st);
   hipLaunchKernelGGL(k_minf<D>, dim3(nb, nchunk), dim3(WAVE), 0, st, h->partial, f, dir, data,
                      h->prb, h->pos, gammas, ngam, chunk, h->nz, h->n, h->S, h->P, h->ox, h->oy,
-                     h->c, cp, cd, model, lagr);
+                     h->c, cp, cd, model, lagr, skip);
   }
   PT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_partials, dim3(nchunk), dim3(64), 0, st, sums, h->partial, nb, ngam);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(nchunk), dim3(64), 0, st, sums, h->partial, nb, ngam,
+                     skip);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
 
 template <int MODE>
 int run_fold(pt_ptycho* h, float2* out, const float2* psi, const float2* hh, const float2* lamd,
-             float rho, hipStream_t st) {
+             float rho, const double* rho_dev, hipStream_t st) {
   long total = (long)h->pt * h->nz * h->n;
   int nb = (int)((total + 255) / 256);
   ProfScope prof("ptycho_fold", st);
   hipLaunchKernelGGL(k_fold<MODE>, dim3(nb), dim3(256), 0, st, out, h->scratch, h->tile_slot, psi,
-                     hh, lamd, rho, h->pt, h->nz, h->n, h->nty, h->ntx, h->B, h->EB);
+                     hh, lamd, rho, rho_dev, h->pt, h->nz, h->n, h->nty, h->ntx, h->B, h->EB);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -512,8 +580,8 @@ int fwd_impl(pt_ptycho* h, void* g, const void* f, float scale, pt_stream stream
 
 template <int MODE>
 int tiles_impl(pt_ptycho* h, float2* out, const float2* gin, const float2* f, const float* data,
-               const float2* hh, const float2* lamd, double rho, float cp, float cd,
-               float out_scale, int model, hipStream_t st) {
+               const float2* hh, const float2* lamd, double rho, const double* rho_dev, float cp,
+               float cd, float out_scale, int model, hipStream_t st) {
   int rc;
   switch (h->D) {
     case 8: rc = run_tiles<8, MODE>(h, gin, f, data, cp, cd, out_scale, model, st); break;
@@ -523,10 +591,10 @@ int tiles_impl(pt_ptycho* h, float2* out, const float2* gin, const float2* f, co
     default: set_error("unsupported detector size"); return PT_ERR_SIZE;
   }
   if (rc) return rc;
-  return run_fold<MODE>(h, out, f, hh, lamd, (float)rho, st);
+  return run_fold<MODE>(h, out, f, hh, lamd, (float)rho, rho_dev, st);
 }
 
-// LDS budget per single-wave workgroup used to size the overlap-add tiles
+// LDS budget of an overlap-add workgroup (4 per CU at 40 KB)
 constexpr size_t kTileLdsBudget = 40 * 1024;
 
 }  // namespace
@@ -552,9 +620,11 @@ extern "C" int pt_ptycho_create(int ptheta, int nz, int n, int ntheta0, int nsca
   h->oy = (dety - nprb) / 2;
   h->c = 1.0f / sqrtf((float)(detx * dety));
   // tile size from the LDS budget
-  const int G = 64 / detx;
-  size_t tbytes = sizeof(float2) * (size_t)G * detx * (detx + 1);
-  long maxe = (long)std::sqrt((double)(kTileLdsBudget - std::min(tbytes, kTileLdsBudget - 4096)) / sizeof(float2));
+  // tile size from the LDS budget: the extended tile takes what the
+  // transpose buffers leave, and at least P+7 pixels
+  const size_t tbytes = tile_lds_bytes(detx, 0);
+  const size_t left = kTileLdsBudget > tbytes + 4096 ? kTileLdsBudget - tbytes : 4096;
+  const long maxe = (long)std::sqrt((double)left / sizeof(float2));
   int EB = (int)std::max<long>(maxe, nprb + 7);
   int B = EB - (nprb - 1);
   h->B = B; h->EB = EB;
@@ -663,7 +733,8 @@ static int adj_impl(pt_ptycho* h, void* f, const void* g, float scale, pt_stream
   PT_REQUIRE(h && g && f, PT_ERR_ARG, "pt_ptycho_adj: null argument");
   PT_REQUIRE(h->ready, PT_ERR_STATE, "pt_ptycho_adj: set_scan_probe not called");
   return tiles_impl<TILE_ADJ>(h, (float2*)f, (const float2*)g, nullptr, nullptr, nullptr,
-                              nullptr, 0.0, 1.0f, 1.0f, h->c * scale, 0, (hipStream_t)stream);
+                              nullptr, 0.0, nullptr, 1.0f, 1.0f, h->c * scale, 0,
+                              (hipStream_t)stream);
 }
 extern "C" int pt_ptycho_adj(pt_ptycho* h, void* f, const void* g, pt_stream stream) {
   return adj_impl(h, f, g, 1.0f, stream);
@@ -674,22 +745,23 @@ extern "C" int pt_ptycho_adj_scaled(pt_ptycho* h, void* f, const void* g, float 
 }
 
 extern "C" int pt_ptycho_grad(pt_ptycho* h, void* grad, const void* psi, const float* data,
-                              const void* hh, const void* lamd, double rho, float coefptycho,
-                              float coefdata, int model, pt_stream stream) {
+                              const void* hh, const void* lamd, double rho, const double* rho_dev,
+                              float coefptycho, float coefdata, int model, pt_stream stream) {
   PT_REQUIRE(h && grad && psi && data && hh && lamd, PT_ERR_ARG, "pt_ptycho_grad: null argument");
   PT_REQUIRE(h->ready, PT_ERR_STATE, "pt_ptycho_grad: set_scan_probe not called");
   PT_REQUIRE(model == PT_MODEL_POISSON || model == PT_MODEL_GAUSSIAN, PT_ERR_ARG,
              "pt_ptycho_grad: bad model");
-  PT_REQUIRE(rho > 0, PT_ERR_ARG, "pt_ptycho_grad: rho must be positive");
+  PT_REQUIRE(rho_dev || rho > 0, PT_ERR_ARG, "pt_ptycho_grad: rho must be positive");
   return tiles_impl<TILE_GRAD>(h, (float2*)grad, nullptr, (const float2*)psi, data,
-                               (const float2*)hh, (const float2*)lamd, rho, coefptycho, coefdata,
+                               (const float2*)hh, (const float2*)lamd, rho, rho_dev, coefptycho,
+                               coefdata,
                                h->c * coefptycho, model, (hipStream_t)stream);
 }
 
 extern "C" int pt_ptycho_minf(pt_ptycho* h, double* sums, const void* psi, const void* dir,
                               const float* data, const double* gammas, int ngam, int chunk,
                               float coefptycho, float coefdata, int model, int lagr,
-                              pt_stream stream) {
+                              const double* skip, pt_stream stream) {
   PT_REQUIRE(h && sums && psi && data && gammas, PT_ERR_ARG, "pt_ptycho_minf: null argument");
   PT_REQUIRE(h->ready, PT_ERR_STATE, "pt_ptycho_minf: set_scan_probe not called");
   PT_REQUIRE(ngam >= 1 && ngam <= PT_MAX_GAM, PT_ERR_ARG, "pt_ptycho_minf: 1 <= ngam <= 8");
@@ -699,5 +771,5 @@ extern "C" int pt_ptycho_minf(pt_ptycho* h, double* sums, const void* psi, const
              "pt_ptycho_minf: bad model");
   hipStream_t st = (hipStream_t)stream;
   PT_DISPATCH_D(h->D, (run_minf<DD>(h, sums, (const float2*)psi, (const float2*)dir, data, gammas,
-                                     ngam, chunk, coefptycho, coefdata, model, lagr, st)));
+                                     ngam, chunk, coefptycho, coefdata, model, lagr, skip, st)));
 }

@@ -72,9 +72,12 @@ int partial_buf(size_t n, double** out) {
 }
 
 template <int NV, typename F>
-__global__ __launch_bounds__(RB) void k_seg_reduce(double* __restrict__ partial, long seglen, F f) {
+__global__ __launch_bounds__(RB) void k_seg_reduce(double* __restrict__ partial, long seglen, F f,
+                                                   const double* __restrict__ skip) {
   __shared__ double sm[NV * 16];
   const int seg = blockIdx.y;
+  // segments whose line search has finished (skip[2*seg+1] != 0) are not evaluated
+  if (skip && skip[2 * seg + 1] != 0.0) return;
   const long base = (long)seg * seglen;
   double v[NV];
 #pragma unroll
@@ -99,9 +102,10 @@ __global__ void k_finish(double* __restrict__ out, const double* __restrict__ pa
 }
 
 __global__ void k_finish_n(double* __restrict__ out, const double* __restrict__ partial,
-                           int nblk, int nv, int nout) {
+                           int nblk, int nv, int nout, const double* __restrict__ skip) {
   const int seg = blockIdx.x, k = threadIdx.x;
   if (k >= nout) return;
+  if (skip && skip[2 * seg + 1] != 0.0) return;
   double s = 0.0;
   for (int b = 0; b < nblk; ++b) s += partial[((long)seg * nblk + b) * nv + k];
   out[seg * nout + k] = s;
@@ -116,7 +120,8 @@ int seg_reduce(double* out, long len, int nseg, F f, hipStream_t st) {
   double* part;
   int rc = partial_buf((size_t)nseg * nblk * NV, &part);
   if (rc) return rc;
-  hipLaunchKernelGGL((k_seg_reduce<NV, F>), dim3(nblk, nseg), dim3(RB), 0, st, part, seglen, f);
+  hipLaunchKernelGGL((k_seg_reduce<NV, F>), dim3(nblk, nseg), dim3(RB), 0, st, part, seglen, f,
+                     (const double*)nullptr);
   PT_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_finish, dim3(nseg), dim3(64), 0, st, out, part, nblk, NV);
   PT_LAUNCH_CHECK();
@@ -126,7 +131,8 @@ int seg_reduce(double* out, long len, int nseg, F f, hipStream_t st) {
 // as seg_reduce with NV = PT_MAX_CAND accumulators, of which the first
 // `nout` are written per segment
 template <typename F>
-int seg_reduce_n(double* out, long len, int nseg, int nout, F f, hipStream_t st) {
+int seg_reduce_n(double* out, long len, int nseg, int nout, F f, hipStream_t st,
+                 const double* skip = nullptr) {
   constexpr int NV = 8;
   PT_REQUIRE(nseg > 0 && len % nseg == 0, PT_ERR_SIZE, "reduction: nseg must divide len");
   const long seglen = len / nseg;
@@ -135,9 +141,10 @@ int seg_reduce_n(double* out, long len, int nseg, int nout, F f, hipStream_t st)
   double* part;
   int rc = partial_buf((size_t)nseg * nblk * NV, &part);
   if (rc) return rc;
-  hipLaunchKernelGGL((k_seg_reduce<NV, F>), dim3(nblk, nseg), dim3(RB), 0, st, part, seglen, f);
+  hipLaunchKernelGGL((k_seg_reduce<NV, F>), dim3(nblk, nseg), dim3(RB), 0, st, part, seglen, f,
+                     skip);
   PT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_finish_n, dim3(nseg), dim3(64), 0, st, out, part, nblk, NV, nout);
+  hipLaunchKernelGGL(k_finish_n, dim3(nseg), dim3(64), 0, st, out, part, nblk, NV, nout, skip);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -158,6 +165,13 @@ int map(long len, F f, hipStream_t st) {
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
+
+// A scalar kernel argument given either by value or by a device address (the
+// device-driven solver keeps rho, tau and the takexi phase in device memory).
+struct SV {
+  double v; const double* p;
+  __device__ double get() const { return p ? p[0] : v; }
+};
 
 // ------------------------------------------------------------------ functors
 struct FDot {
@@ -270,8 +284,10 @@ struct FRegAdj {
 };
 // solve_reg (solver.py:257-264)
 struct FRegSolve {
-  float2* out; const float2* u; const float2* mu; float itau, thr; int nz, ny, nx;
+  float2* out; const float2* u; const float2* mu; SV tau; double alpha; int nz, ny, nx;
   __device__ void operator()(long i) const {
+    const double t = tau.get();
+    const float itau = (float)(1.0 / t), thr = (float)(alpha / t);
     const long plane = (long)nz * ny * nx;
     const int x = (int)(i % nx);
     const long r = i / nx;
@@ -305,14 +321,16 @@ struct FRegSolve {
 };
 
 struct FExpTomo {
-  float2* out; const float2* psi; float c; float2 ph;
+  float2* out; const float2* psi; float c; float2 ph; const double* ph_dev;
   __device__ void operator()(long i) const {
+    // device phase: ph_dev = {pshift, cos(pshift), sin(pshift)}
+    const float2 phv = ph_dev ? make_float2((float)ph_dev[1], (float)ph_dev[2]) : ph;
     // exp(i*c*psi) = exp(-c*Im psi) * (cos(c*Re psi) + i sin(c*Re psi))
     const float2 p = psi[i];
     const float a = expf(-c * p.y);
     float sn, cs;
     sincosf(c * p.x, &sn, &cs);
-    out[i] = cmul(make_float2(a * cs, a * sn), ph);
+    out[i] = cmul(make_float2(a * cs, a * sn), phv);
   }
 };
 
@@ -328,8 +346,9 @@ struct FMlog {
 };
 
 struct FTakexiStats {
-  const float2* psi; const float2* lamd; float irho; int nz, n, r;
+  const float2* psi; const float2* lamd; SV rho; int nz, n, r;
   __device__ void operator()(long i, double (&v)[3]) const {
+    const float irho = (float)(1.0 / rho.get());
     const int x = (int)(i % n);
     const float2 p = psi[i];
     if (x >= r && x < 2 * r) v[0] += (double)atan2f(p.y, p.x);
@@ -342,9 +361,13 @@ struct FTakexiStats {
 };
 
 struct FTakexi {
-  float2* xi0; float2* K; const float2* psi; const float2* lamd; float rho; float2 eph;
-  float kfac; float xfac;
+  float2* xi0; float2* K; const float2* psi; const float2* lamd; SV rhov; float2 ephv;
+  float kfacv; float xfac; const double* prm;
   __device__ void operator()(long i) const {
+    // device parameters: prm = {pshift, cos(pshift), sin(pshift), kfac}
+    const float rho = (float)rhov.get();
+    const float2 eph = prm ? make_float2((float)prm[1], -(float)prm[2]) : ephv;
+    const float kfac = prm ? (float)prm[3] : kfacv;
     const float2 p = psi[i], l = lamd[i];
     float2 t = make_float2(p.x - l.x / rho, p.y - l.y / rho);
     t = cmul(t, eph);
@@ -361,11 +384,11 @@ struct FTakexi {
 
 // cg_tomo residual: r = conj(K)*(K*Ru - xi0); v[0] += |K*Ru - xi0|^2
 struct FTomoResid {
-  float2* r; const float2* K; const float2* Ru; const float2* xi0;
+  float2* r; const float2* K; const float2* Ru; const float2* xi0; SV scale;
   __device__ void operator()(long i, double (&v)[1]) const {
     const float2 k = K[i];
     const float2 e = csub(cmul(k, Ru[i]), xi0[i]);
-    r[i] = cmul(cconj(k), e);
+    r[i] = cscale(cmul(cconj(k), e), (float)scale.get());
     v[0] += (double)e.x * e.x + (double)e.y * e.y;
   }
 };
@@ -382,7 +405,7 @@ __device__ __forceinline__ void grad3(const float2* __restrict__ u, long i, int 
 
 // grad = rho_part + tau * J*(J u - xi1);  v[0] += |J u - xi1|^2
 struct FRegGrad {
-  float2* grad; const float2* u; const float2* xi1; float tau; int nz, ny, nx;
+  float2* grad; const float2* u; const float2* xi1; SV tau; int nz, ny, nx;
   __device__ void operator()(long i, double (&v)[1]) const {
     const long plane = (long)nz * ny * nx;
     const int x = (int)(i % nx);
@@ -412,7 +435,7 @@ struct FRegGrad {
       grad3(u, i - off, x, y, z - 1, nz, ny, nx, gn);
       s = csub(s, csub(gn[2], xi1[2 * plane + i - off]));
     }
-    const float c = -1.15470053837925152902f * tau;
+    const float c = -1.15470053837925152902f * (float)tau.get();
     const float2 gv = grad[i];
     grad[i] = make_float2(gv.x + c * s.x, gv.y + c * s.y);
   }
@@ -423,10 +446,11 @@ struct FRegGrad {
 #define PT_MAX_CAND 8
 // cg_ptycho rho term (solver.py:213): |h - (psi + g*d) + lamd/rho|^2
 struct FRhoMinf {
-  const float2* h; const float2* psi; const float2* lamd; const float2* d; float rho;
+  const float2* h; const float2* psi; const float2* lamd; const float2* d; SV rhov;
   const double* gam; int ngam; long seglen;
   __device__ void operator()(long i, double (&v)[PT_MAX_CAND]) const {
     const long s = i / seglen;
+    const float rho = (float)rhov.get();
     const float2 a = h[i], p = psi[i], l = lamd[i], dd = d[i];
     const float lx = l.x / rho, ly = l.y / rho;
 #pragma unroll
@@ -516,6 +540,18 @@ struct FLincomb3 {
     z[i] = r;
   }
 };
+// ADMM dual update (solver.py:326-327): out = lam + k*(a - b), k = c or 1/c;
+// a may be null (0), which gives xi1 = phi - mu/tau (solver.py:168)
+struct FDual {
+  float2* out; const float2* lam; const float2* a; const float2* b; SV c; int inv;
+  __device__ void operator()(long i) const {
+    const double cv = c.get();
+    const float cf = (float)(inv ? 1.0 / cv : cv);
+    const float2 av = a ? a[i] : make_float2(0.f, 0.f);
+    const float2 l = lam[i], t = csub(av, b[i]);
+    out[i] = make_float2(l.x + cf * t.x, l.y + cf * t.y);
+  }
+};
 struct FAxpySeg {
   float2* y; const double* a; const float2* x; long seglen;
   __device__ void operator()(long i) const {
@@ -581,20 +617,22 @@ extern "C" int pt_k_reg_adj(void* out, const void* g, int nz, int ny, int nx, pt
   return map((long)nz * ny * nx, FRegAdj{(float2*)out, (const float2*)g, nz, ny, nx},
              (hipStream_t)stream);
 }
-extern "C" int pt_k_reg_solve(void* out, const void* u, const void* mu, double tau, double alpha,
-                              int nz, int ny, int nx, pt_stream stream) {
+extern "C" int pt_k_reg_solve(void* out, const void* u, const void* mu, double tau,
+                              const double* tau_dev, double alpha, int nz, int ny, int nx,
+                              pt_stream stream) {
   PT_REQUIRE(out && u && mu, PT_ERR_ARG, "pt_k_reg_solve: null argument");
-  PT_REQUIRE(tau > 0, PT_ERR_ARG, "pt_k_reg_solve: tau must be positive");
+  PT_REQUIRE(tau_dev || tau > 0, PT_ERR_ARG, "pt_k_reg_solve: tau must be positive");
   return map((long)nz * ny * nx,
-             FRegSolve{(float2*)out, (const float2*)u, (const float2*)mu, (float)(1.0 / tau),
-                       (float)(alpha / tau), nz, ny, nx},
+             FRegSolve{(float2*)out, (const float2*)u, (const float2*)mu, SV{tau, tau_dev}, alpha,
+                       nz, ny, nx},
              (hipStream_t)stream);
 }
-extern "C" int pt_k_exptomo(void* out, const void* psi, double c, double phase, long len,
-                            pt_stream stream) {
+extern "C" int pt_k_exptomo(void* out, const void* psi, double c, double phase,
+                            const double* phase_dev, long len, pt_stream stream) {
   PT_REQUIRE(out && psi, PT_ERR_ARG, "pt_k_exptomo: null argument");
   const float2 ph = make_float2((float)std::cos(phase), (float)std::sin(phase));
-  return map(len, FExpTomo{(float2*)out, (const float2*)psi, (float)c, ph}, (hipStream_t)stream);
+  return map(len, FExpTomo{(float2*)out, (const float2*)psi, (float)c, ph, phase_dev},
+             (hipStream_t)stream);
 }
 extern "C" int pt_k_mlog(void* out, const void* x, long len, pt_stream stream) {
   PT_REQUIRE(out && x, PT_ERR_ARG, "pt_k_mlog: null argument");
@@ -640,16 +678,18 @@ __global__ void k_takexi_finish(double* out, const double* partial, int nblk) {
 }  // namespace pt
 
 extern "C" int pt_k_takexi_stats(double* out, const void* psi, const void* lamd, double rho,
-                                 int ntheta, int nz, int n, int r, pt_stream stream) {
+                                 const double* rho_dev, int ntheta, int nz, int n, int r,
+                                 pt_stream stream) {
   PT_REQUIRE(out && psi && lamd, PT_ERR_ARG, "pt_k_takexi_stats: null argument");
-  PT_REQUIRE(rho > 0 && r >= 0 && 2 * r <= n, PT_ERR_ARG, "pt_k_takexi_stats: bad rho or r");
+  PT_REQUIRE((rho_dev || rho > 0) && r >= 0 && 2 * r <= n, PT_ERR_ARG,
+             "pt_k_takexi_stats: bad rho or r");
   hipStream_t st = (hipStream_t)stream;
   const long len = (long)ntheta * nz * n;
   const int nblk = (int)std::min<long>(1024, std::max<long>(1, (len + RB * 8 - 1) / (RB * 8)));
   double* part;
   int rc = partial_buf((size_t)nblk * 3, &part);
   if (rc) return rc;
-  FTakexiStats f{(const float2*)psi, (const float2*)lamd, (float)(1.0 / rho), nz, n, r};
+  FTakexiStats f{(const float2*)psi, (const float2*)lamd, SV{rho, rho_dev}, nz, n, r};
   hipLaunchKernelGGL(k_takexi_stats, dim3(nblk), dim3(RB), 0, st, part, f, len);
   PT_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_takexi_finish, dim3(1), dim3(64), 0, st, out, part, nblk);
@@ -666,8 +706,44 @@ extern "C" int pt_k_takexi(void* xi0, void* K, const void* psi, const void* lamd
   // K = i*kc*t*kscale ; xi0 = K*(-i*logt/c1)*coeftomo
   return map(len,
              FTakexi{(float2*)xi0, (float2*)K, (const float2*)psi, (const float2*)lamd,
-                     (float)rho, eph, (float)(kc * kscale), (float)(coeftomo / c1)},
+                     SV{rho, nullptr}, eph, (float)(kc * kscale), (float)(coeftomo / c1),
+                     nullptr},
              (hipStream_t)stream);
+}
+
+namespace pt {
+namespace {
+// takexi scalars from the statistics pass, as solver.py:149-168 forms them:
+// pshift = float32 mean of the two column-band angle means, kscale =
+// 1/(kc*max|t|) (0 if max|t| = 0).  prm = {pshift, cos, sin, kc*kscale}.
+__global__ void k_takexi_params(double* prm, const double* stats, double cnt, double kc) {
+  if (threadIdx.x) return;
+  const float m1 = (float)(stats[0] / cnt), m2 = (float)(stats[1] / cnt);
+  const double pshift = (double)((m1 + m2) / 2.0f);
+  const double tmax = stats[2];
+  const double kscale = tmax > 0 ? 1.0 / (kc * tmax) : 0.0;
+  prm[0] = pshift;
+  prm[1] = cos(pshift);
+  prm[2] = sin(pshift);
+  prm[3] = kc * kscale;
+}
+}  // namespace
+}  // namespace pt
+
+extern "C" int pt_k_takexi_dev(void* xi0, void* K, double* prm, const void* psi,
+                               const void* lamd, const double* rho_dev, const double* stats,
+                               double cnt, double kc, double c1, double coeftomo, long len,
+                               pt_stream stream) {
+  PT_REQUIRE(xi0 && K && prm && psi && lamd && rho_dev && stats, PT_ERR_ARG,
+             "pt_k_takexi_dev: null argument");
+  PT_REQUIRE(cnt > 0, PT_ERR_ARG, "pt_k_takexi_dev: cnt must be positive");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_takexi_params, dim3(1), dim3(64), 0, st, prm, stats, cnt, kc);
+  PT_LAUNCH_CHECK();
+  return map(len,
+             FTakexi{(float2*)xi0, (float2*)K, (const float2*)psi, (const float2*)lamd,
+                     SV{0.0, rho_dev}, make_float2(0.f, 0.f), 0.f, (float)(coeftomo / c1), prm},
+             st);
 }
 
 namespace pt {
@@ -701,20 +777,29 @@ __global__ void k_max_finish(double* out, const double* partial, int nblk) {
 }  // namespace pt
 
 extern "C" int pt_k_tomo_resid(void* r, double* out, const void* K, const void* Ru,
-                               const void* xi0, long len, pt_stream stream) {
+                               const void* xi0, double scale, const double* scale_dev, long len,
+                               pt_stream stream) {
   PT_REQUIRE(r && out && K && Ru && xi0, PT_ERR_ARG, "pt_k_tomo_resid: null argument");
   return seg_reduce<1>(out, len, 1,
                        FTomoResid{(float2*)r, (const float2*)K, (const float2*)Ru,
-                                  (const float2*)xi0},
+                                  (const float2*)xi0, SV{scale, scale_dev}},
                        (hipStream_t)stream);
 }
 extern "C" int pt_k_reg_grad(void* grad, double* out, const void* u, const void* xi1, double tau,
-                             int nz, int ny, int nx, pt_stream stream) {
+                             const double* tau_dev, int nz, int ny, int nx, pt_stream stream) {
   PT_REQUIRE(grad && out && u && xi1, PT_ERR_ARG, "pt_k_reg_grad: null argument");
   return seg_reduce<1>(out, (long)nz * ny * nx, 1,
-                       FRegGrad{(float2*)grad, (const float2*)u, (const float2*)xi1, (float)tau,
-                                nz, ny, nx},
+                       FRegGrad{(float2*)grad, (const float2*)u, (const float2*)xi1,
+                                SV{tau, tau_dev}, nz, ny, nx},
                        (hipStream_t)stream);
+}
+extern "C" int pt_k_dual(void* out, const void* lam, const void* a, const void* b, double c,
+                         const double* c_dev, int inv, long len, pt_stream stream) {
+  PT_REQUIRE(out && lam && b, PT_ERR_ARG, "pt_k_dual: null argument");
+  return map(len,
+             FDual{(float2*)out, (const float2*)lam, (const float2*)a, (const float2*)b,
+                   SV{c, c_dev}, inv},
+             (hipStream_t)stream);
 }
 extern "C" int pt_k_dot_diff(double* out, const void* a, const void* b, const void* c, long len,
                              pt_stream stream) {
@@ -802,34 +887,165 @@ extern "C" int pt_prof_stop(double* total_ms, long* launches) {
 }
 
 extern "C" int pt_k_rho_minf(double* out, const void* h, const void* psi, const void* lamd,
-                             const void* d, double rho, const double* gammas, int ngam, long len,
-                             int nseg, pt_stream stream) {
+                             const void* d, double rho, const double* rho_dev,
+                             const double* gammas, int ngam, long len, int nseg,
+                             const double* skip, pt_stream stream) {
   PT_REQUIRE(out && h && psi && lamd && d && gammas, PT_ERR_ARG, "pt_k_rho_minf: null argument");
-  PT_REQUIRE(rho > 0, PT_ERR_ARG, "pt_k_rho_minf: rho must be positive");
+  PT_REQUIRE(rho_dev || rho > 0, PT_ERR_ARG, "pt_k_rho_minf: rho must be positive");
   PT_REQUIRE(ngam >= 1 && ngam <= PT_MAX_CAND, PT_ERR_ARG, "pt_k_rho_minf: 1 <= ngam <= 8");
   PT_REQUIRE(nseg > 0 && len % nseg == 0, PT_ERR_SIZE, "pt_k_rho_minf: nseg must divide len");
   return seg_reduce_n(out, len, nseg, ngam,
                       FRhoMinf{(const float2*)h, (const float2*)psi, (const float2*)lamd,
-                               (const float2*)d, (float)rho, gammas, ngam, len / nseg},
-                      (hipStream_t)stream);
+                               (const float2*)d, SV{rho, rho_dev}, gammas, ngam, len / nseg},
+                      (hipStream_t)stream, skip);
 }
 extern "C" int pt_k_tomo_minf(double* out, const void* K, const void* Ru, const void* Rd,
                               const void* xi0, const double* gammas, int ngam, long len,
-                              pt_stream stream) {
+                              const double* skip, pt_stream stream) {
   PT_REQUIRE(out && K && Ru && Rd && xi0 && gammas, PT_ERR_ARG, "pt_k_tomo_minf: null argument");
   PT_REQUIRE(ngam >= 1 && ngam <= PT_MAX_CAND, PT_ERR_ARG, "pt_k_tomo_minf: 1 <= ngam <= 8");
   return seg_reduce_n(out, len, 1, ngam,
                       FTomoMinf{(const float2*)K, (const float2*)Ru, (const float2*)Rd,
                                 (const float2*)xi0, gammas, ngam},
-                      (hipStream_t)stream);
+                      (hipStream_t)stream, skip);
 }
 extern "C" int pt_k_reg_minf(double* out, const void* u, const void* d, const void* xi1,
                              const double* gammas, int ngam, int nz, int ny, int nx,
-                             pt_stream stream) {
+                             const double* skip, pt_stream stream) {
   PT_REQUIRE(out && u && d && xi1 && gammas, PT_ERR_ARG, "pt_k_reg_minf: null argument");
   PT_REQUIRE(ngam >= 1 && ngam <= PT_MAX_CAND, PT_ERR_ARG, "pt_k_reg_minf: 1 <= ngam <= 8");
   return seg_reduce_n(out, (long)nz * ny * nx, 1, ngam,
                       FRegMinf{(const float2*)u, (const float2*)d, (const float2*)xi1, gammas,
                                ngam, nz, ny, nx},
-                      (hipStream_t)stream);
+                      (hipStream_t)stream, skip);
+}
+
+// ------------------------------------------------------------------ device-side decisions
+// The scalar steps of the CG and ADMM loops, computed on the device so that an
+// ADMM iteration enqueues without host round trips (and can be captured as a
+// graph).  FP64, no contraction: the host restatement of each rule gives the
+// same bits.
+namespace pt {
+namespace {
+#pragma clang fp contract(off)
+// Dai-Yuan (solver.py:195-196, 230-231): beta = ||g||^2 / sum(conj(d)(g - g0)),
+// red[s] = {||g||^2, Re den, Im den}; beta = 0 when den is 0 or not finite
+// (a zero step leaves g == g0, where the reference divides 0 by 0).
+__global__ void k_dai_yuan(double* __restrict__ beta, const double* __restrict__ red, int nseg) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  const double gn = red[3 * s], dr = red[3 * s + 1], di = red[3 * s + 2];
+  const double den2 = dr * dr + di * di;
+  double br = 0.0, bi = 0.0;
+  if (den2 > 0.0 && isfinite(den2)) {
+    br = gn * dr / den2;
+    bi = -(gn * di) / den2;
+    if (!isfinite(br) || !isfinite(bi)) br = bi = 0.0;
+  }
+  beta[2 * s] = br;
+  beta[2 * s + 1] = bi;
+}
+
+// Halving line search (solver.py:171-177) for nseg independent segments, as
+// sweeps of ngam candidates: cand[s][0] = 0, cand[s][j] = cur * 2^-(j-1).
+// state[s] = {cur, done}; step[s] = {gamma, 0} once done.
+__global__ void k_ls_begin(double* __restrict__ state, double* __restrict__ cand, int nseg,
+                           int ngam) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  const double cur = state[2 * s];
+  state[2 * s + 1] = 0.0;
+  cand[s * ngam] = 0.0;
+  for (int j = 1; j < ngam; ++j) cand[s * ngam + j] = cur * ldexp(1.0, -(j - 1));
+}
+
+// f[s][j] = w[0]*f1[s][j] + w[1]*f2[s][j] (f2 may be null).  A segment accepts
+// the first candidate that is not a strict ascent or is <= 1e-12 (then 0);
+// otherwise its next sweep continues from half the last candidate.  With
+// `last`, an unfinished segment ends at 0 as the halving rule would.
+__global__ void k_ls_select(double* __restrict__ state, double* __restrict__ cand,
+                            double* __restrict__ step, const double* __restrict__ f1,
+                            const double* __restrict__ f2, const double* __restrict__ w, int nseg,
+                            int ngam, int last) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  if (state[2 * s + 1] != 0.0) return;
+  const double w0 = w[0], w1 = f2 ? w[1] : 0.0;
+  const double* a = f1 + s * ngam;
+  const double* b = f2 ? f2 + s * ngam : nullptr;
+  const double fz = w0 * a[0] + (b ? w1 * b[0] : 0.0);
+  for (int j = 1; j < ngam; ++j) {
+    const double g = cand[s * ngam + j];
+    const double fj = w0 * a[j] + (b ? w1 * b[j] : 0.0);
+    if (fz - fj < 0.0 && g > 1e-12) continue;
+    const double out = g <= 1e-12 ? 0.0 : g;
+    step[2 * s] = out;
+    step[2 * s + 1] = 0.0;
+    state[2 * s] = out;
+    state[2 * s + 1] = 1.0;
+    return;
+  }
+  const double cur = cand[s * ngam + ngam - 1] * 0.5;
+  if (last) {
+    step[2 * s] = step[2 * s + 1] = 0.0;
+    state[2 * s] = 0.0;
+    state[2 * s + 1] = 1.0;
+    return;
+  }
+  state[2 * s] = cur;
+  cand[s * ngam] = 0.0;
+  for (int j = 1; j < ngam; ++j) cand[s * ngam + j] = cur * ldexp(1.0, -(j - 1));
+}
+
+// Residual balancing (solver.py:267-282).  norms = {||psi - h||^2,
+// ||h - h0||^2, ||phi - e||^2, ||e - e0||^2}; the reference's
+// ||rho (h - h0)||^2 is rho^2 ||h - h0||^2 (rho is a power of 2, so the
+// scaling is exact).  rt = {rho, tau}, updated in place.
+__global__ void k_penalty(double* __restrict__ rt, const double* __restrict__ norms) {
+  if (threadIdx.x) return;
+  double rho = rt[0], tau = rt[1];
+  const double r = norms[0], s = rho * rho * norms[1];
+  const double r2 = norms[2], s2 = tau * tau * norms[3];
+  if (r > 10.0 * s) rho *= 2.0;
+  else if (s > 10.0 * r) rho *= 0.5;
+  if (r2 > 10.0 * s2) tau *= 2.0;
+  else if (s2 > 10.0 * r2) tau *= 0.5;
+  rt[0] = rho;
+  rt[1] = tau;
+}
+#pragma clang fp contract(on)
+}  // namespace
+}  // namespace pt
+
+extern "C" int pt_k_dai_yuan(double* beta, const double* red, int nseg, pt_stream stream) {
+  PT_REQUIRE(beta && red && nseg > 0, PT_ERR_ARG, "pt_k_dai_yuan: bad argument");
+  hipLaunchKernelGGL(k_dai_yuan, dim3((nseg + 63) / 64), dim3(64), 0, (hipStream_t)stream, beta,
+                     red, nseg);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+extern "C" int pt_k_ls_begin(double* state, double* cand, int nseg, int ngam, pt_stream stream) {
+  PT_REQUIRE(state && cand && nseg > 0, PT_ERR_ARG, "pt_k_ls_begin: bad argument");
+  PT_REQUIRE(ngam >= 2 && ngam <= PT_MAX_CAND, PT_ERR_ARG, "pt_k_ls_begin: 2 <= ngam <= 8");
+  hipLaunchKernelGGL(k_ls_begin, dim3((nseg + 63) / 64), dim3(64), 0, (hipStream_t)stream, state,
+                     cand, nseg, ngam);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+extern "C" int pt_k_ls_select(double* state, double* cand, double* step, const double* f1,
+                              const double* f2, const double* w, int nseg, int ngam, int last,
+                              pt_stream stream) {
+  PT_REQUIRE(state && cand && step && f1 && w && nseg > 0, PT_ERR_ARG,
+             "pt_k_ls_select: bad argument");
+  PT_REQUIRE(ngam >= 2 && ngam <= PT_MAX_CAND, PT_ERR_ARG, "pt_k_ls_select: 2 <= ngam <= 8");
+  hipLaunchKernelGGL(k_ls_select, dim3((nseg + 63) / 64), dim3(64), 0, (hipStream_t)stream, state,
+                     cand, step, f1, f2, w, nseg, ngam, last);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+extern "C" int pt_k_penalty(double* rho_tau, const double* norms, pt_stream stream) {
+  PT_REQUIRE(rho_tau && norms, PT_ERR_ARG, "pt_k_penalty: null argument");
+  hipLaunchKernelGGL(k_penalty, dim3(1), dim3(64), 0, (hipStream_t)stream, rho_tau, norms);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
 }

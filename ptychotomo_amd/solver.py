@@ -20,7 +20,11 @@ in rounding only):
   element-wise in FP32 -- so a step below FP32 resolution reproduces the
   gamma = 0 value exactly and the halving rule stops where the reference's
   does instead of collapsing to gamma = 0.
-* Scalars (norms, dots, steps) are accumulated in FP64.
+* Scalars (norms, dots, steps) are accumulated in FP64, and the scalar rules
+  (Dai-Yuan beta, the halving decisions, the takexi phase, the penalty
+  update) run as FP64 device kernels, so an ADMM iteration is enqueued
+  without host round trips and can be replayed as a HIP graph.  After a zero
+  step the reference's Dai-Yuan beta is 0/0; here it is 0 (restart).
 """
 import ctypes
 import warnings
@@ -37,24 +41,15 @@ from .radonusfft import radonusfft
 PLANCK_CONSTANT = 6.58211928e-19  # [keV*s]   solver.py:15
 SPEED_OF_LIGHT = 299792458e+2     # [cm/s]    solver.py:16
 
-_MAXGAM = 8
+_MAXGAM = 8      # line-search candidates per sweep (gamma = 0 and 7 halvings)
+_SWEEPS = 6      # sweeps per line search: 42 halvings take 2 below 1e-12
+# device scalar block (FP64): 1, rho, tau, takexi {pshift, cos, sin, kfac},
+# takexi statistics, penalty norms, max |angle psi|
+_ONE, _RHO, _TAU, _PRM, _STATS, _NORMS, _MAXANG, _NSC = 0, 1, 2, 3, 8, 12, 16, 24
 
 
 def _call(name, *args):
     return _lib.call(name, *args)
-
-
-def _dai_yuan(gnorm2, den):
-    """Dai-Yuan beta = ||g||^2 / sum(conj(d)(g - g0)) (solver.py:195-196,
-    230-231, complex).  After a zero step (the halving rule found no descent)
-    g == g0 and the reference divides 0 by 0; here beta = 0 instead, i.e. a
-    steepest-descent restart, so the iterate stays finite."""
-    den = np.asarray(den, np.complex128)
-    ok = (den != 0) & np.isfinite(den)
-    beta = np.zeros(den.shape, np.complex128)
-    beta[ok] = np.asarray(gnorm2, np.float64)[ok] / den[ok]
-    beta[~np.isfinite(beta)] = 0
-    return beta
 
 
 class Solver(object):
@@ -102,6 +97,7 @@ class Solver(object):
         self.coefdata = np.float32(1) / (np.float32(self.ndetx * self.ndety)
                                          * (np.abs(prb_h) ** 2).max().astype(np.float32))
         self._sc = torch.zeros(64, dtype=f64, device=device())  # scalar results
+        self._works = {}
         self._data_cache = None
         self.history = []
         self.state = None
@@ -140,7 +136,7 @@ class Solver(object):
         psi = to_dev(psi, c64)
         res = empty(psi.shape)
         c = self.voxelsize * self.wavenumber() / float(self.coeftomo)
-        _call("pt_k_exptomo", ptr(res), ptr(psi), c, 0.0, psi.numel(), stream())
+        _call("pt_k_exptomo", ptr(res), ptr(psi), c, 0.0, None, psi.numel(), stream())
         return res
 
     def logtomo(self, psi):
@@ -249,34 +245,55 @@ class Solver(object):
         u = to_dev(u, c64)
         mu = to_dev(mu, c64)
         res = empty(mu.shape)
-        _call("pt_k_reg_solve", ptr(res), ptr(u), ptr(mu), float(tau), float(alpha), *u.shape,
+        _call("pt_k_reg_solve", ptr(res), ptr(u), ptr(mu), float(tau), None, float(alpha), *u.shape,
               stream())
         return res
 
+    # ------------------------------------------------------------------ device scalars
+    def _scalar_block(self, rho=0.5, tau=0.5):
+        """FP64 device block holding the ADMM scalars (layout _ONE.._NSC)."""
+        blk = torch.zeros(_NSC, dtype=f64)
+        blk[_ONE], blk[_RHO], blk[_TAU] = 1.0, float(rho), float(tau)
+        return blk.to(device())
+
+    def _work(self, key, nseg, shape):
+        """Per-shape CG workspace (allocated once, reused: stable addresses
+        for graph replay)."""
+        w = self._works.get(key)
+        if w is None or w["shape"] != tuple(shape) or w["nseg"] != nseg:
+            z = lambda n: torch.zeros(n, dtype=f64, device=device())
+            w = {"shape": tuple(shape), "nseg": nseg,
+                 "red": z(3 * nseg), "beta": z(2 * nseg), "state": z(2 * nseg),
+                 "step": z(2 * nseg), "cand": z(_MAXGAM * nseg), "f1": z(_MAXGAM * nseg),
+                 "f2": z(_MAXGAM * nseg), "sc": z(4),
+                 "state0": torch.tensor([2.0, 0.0] * nseg, dtype=f64, device=device()),
+                 "grad": [empty(shape), empty(shape)], "d": empty(shape)}
+            self._works[key] = w
+        return w
+
     # ------------------------------------------------------------------ takexi
     def takexi(self, psi, phi, lamd, mu, rho, tau):
-        """solver.py:149-168 (r = nprb // 2)."""
-        psi = to_dev(psi, c64)
-        lamd = to_dev(lamd, c64)
-        phi = to_dev(phi, c64)
-        mu = to_dev(mu, c64)
+        """solver.py:149-168 (r = nprb // 2).  Returns (xi0, xi1, K, pshift)."""
+        psi, lamd = to_dev(psi, c64), to_dev(lamd, c64)
+        phi, mu = to_dev(phi, c64), to_dev(mu, c64)
+        blk = self._scalar_block(rho, tau)
+        xi0, K, xi1 = empty(psi.shape), empty(psi.shape), empty(phi.shape)
+        self._takexi_dev(psi, phi, lamd, mu, blk, xi0, K, xi1)
+        return xi0, xi1, K, float(blk[_PRM].item())
+
+    def _takexi_dev(self, psi, phi, lamd, mu, blk, xi0, K, xi1):
+        st = stream()
         r = self.nprb // 2
-        _call("pt_k_takexi_stats", ptr(self._sc), ptr(psi), ptr(lamd), float(rho), self.ntheta,
-              self.nz, self.n, r, stream())
-        s1, s2, tmax = self._scalars(3)
-        cnt = self.ntheta * self.nz * r
-        pshift = float(np.float32((np.float32(s1 / cnt) + np.float32(s2 / cnt)) / 2))
-        kc = self.voxelsize * self.wavenumber() / float(self.coeftomo)
-        kscale = 1.0 / (kc * tmax) if tmax > 0 else 0.0
-        xi0 = empty(psi.shape)
-        K = empty(psi.shape)
-        _call("pt_k_takexi", ptr(xi0), ptr(K), ptr(psi), ptr(lamd), float(rho), pshift, kc,
-              kscale, self.voxelsize * self.wavenumber(), float(self.coeftomo), psi.numel(),
-              stream())
-        xi1 = empty(phi.shape)
-        _call("pt_k_lincomb3", ptr(xi1), 1.0, 0.0, ptr(phi), -1.0 / tau, 0.0, ptr(mu), 0.0, 0.0,
-              None, phi.numel(), stream())
-        return xi0, xi1, K, pshift
+        _call("pt_k_takexi_stats", _dp(blk, _STATS), ptr(psi), ptr(lamd), 0.0, _dp(blk, _RHO),
+              self.ntheta, self.nz, self.n, r, st)
+        c1 = self.voxelsize * self.wavenumber()
+        kc = c1 / float(self.coeftomo)
+        _call("pt_k_takexi_dev", ptr(xi0), ptr(K), _dp(blk, _PRM), ptr(psi), ptr(lamd),
+              _dp(blk, _RHO), _dp(blk, _STATS), float(self.ntheta * self.nz * r), kc, c1,
+              float(self.coeftomo), psi.numel(), st)
+        # xi1 = phi - mu/tau
+        _call("pt_k_dual", ptr(xi1), ptr(phi), None, ptr(mu), 0.0, _dp(blk, _TAU), 1,
+              phi.numel(), st)
 
     # ------------------------------------------------------------------ line search
     def line_search(self, minf, gamma, u, fu, d, fd):
@@ -287,198 +304,177 @@ class Solver(object):
             gamma = 0
         return gamma
 
-    @staticmethod
-    def _halving_sweeps(gamma, evalf):
-        """Halving rule of line_search (solver.py:171-177) for a vector of
-        independent segments.  ``evalf(cand)`` returns the functional at the
-        candidate steps cand[s, j] (column 0 is gamma = 0); each sweep tries
-        the current step and its next 6 halvings for every segment."""
-        nseg = len(gamma)
-        cur = np.array(gamma, dtype=np.float64)
-        out = np.zeros(nseg)
-        done = np.zeros(nseg, bool)
-        while not done.all():
-            cand = np.zeros((nseg, _MAXGAM))
-            cand[:, 1:] = cur[:, None] * 0.5 ** np.arange(_MAXGAM - 1)[None, :]
-            f = evalf(cand)
-            for s in range(nseg):
-                if done[s]:
-                    continue
-                for j in range(1, _MAXGAM):
-                    g = cand[s, j]
-                    if (f[s, 0] - f[s, j] < 0) and g > 1e-12:
-                        continue
-                    out[s] = 0.0 if g <= 1e-12 else g
-                    done[s] = True
-                    break
-                else:
-                    cur[s] = cand[s, -1] * 0.5
-        return out
+    def _line_search_dev(self, w, nseg, evalf, weights):
+        """The halving rule of line_search for nseg independent segments, on
+        the device: sweeps of 7 halvings (plus gamma = 0) until every segment
+        has accepted a step (_SWEEPS sweeps cover 2 -> 1e-12).  evalf(cand,
+        state, f1, f2) enqueues the two functional terms at the candidates,
+        skipping finished segments; weights points to their two FP64
+        weights."""
+        st = stream()
+        _call("pt_k_ls_begin", ptr(w["state"]), ptr(w["cand"]), nseg, _MAXGAM, st)
+        for sw in range(_SWEEPS):
+            evalf(w["cand"], w["state"], w["f1"], w["f2"])
+            _call("pt_k_ls_select", ptr(w["state"]), ptr(w["cand"]), ptr(w["step"]),
+                  ptr(w["f1"]), ptr(w["f2"]), weights, nseg, _MAXGAM, int(sw == _SWEEPS - 1), st)
+
+    def _dai_yuan_dev(self, w, i, grad, grad0, L, nseg):
+        """d = -grad (i = 0) or -grad + beta d (solver.py:192-196, 227-231)."""
+        st = stream()
+        d = w["d"]
+        if i == 0:
+            _call("pt_k_dy_update", ptr(d), ptr(grad), None, 1, L, nseg, st)
+        else:
+            _call("pt_k_dy_reduce", ptr(w["red"]), ptr(grad), ptr(grad0), ptr(d), L, nseg, st)
+            _call("pt_k_dai_yuan", ptr(w["beta"]), ptr(w["red"]), nseg, st)
+            _call("pt_k_dy_update", ptr(d), ptr(grad), ptr(w["beta"]), 0, L, nseg, st)
 
     # ------------------------------------------------------------------ cg_tomo
     def cg_tomo(self, xi0, xi1, K, init, rho, tau, titer):
         """solver.py:180-203."""
-        u, _ = self._cg_tomo(xi0, xi1, K, init, rho, tau, titer)
-        return u
-
-    def _cg_tomo(self, xi0, xi1, K, init, rho, tau, titer, Ru=None):
         xi0, xi1, K = to_dev(xi0, c64), to_dev(xi1, c64), to_dev(K, c64)
         u = to_dev(init, c64).clone()
+        Ru = self.fwd_tomo_batch(u)
+        blk = self._scalar_block(rho, tau)
+        hist = torch.zeros(max(titer, 1), dtype=f64, device=device())
+        self._cg_tomo_dev(xi0, xi1, K, u, Ru, blk, titer, hist)
+        self._last_tomo_gammas = [float(v) for v in hist[:titer].cpu().numpy()]
+        return u
+
+    def _cg_tomo_dev(self, xi0, xi1, K, u, Ru, blk, titer, hist=None):
+        """cg_tomo on u in place; Ru = R u is carried by linearity
+        (R(u + g d) = R u + g R d), so a CG iteration costs one adjoint and one
+        forward USFFT."""
         nz, n = self.nz, self.n
-        if Ru is None:
-            Ru = self.fwd_tomo_batch(u)
-        else:
-            Ru = Ru.clone()
-        lenp = Ru.numel()
-        lenu = u.numel()
-        sc = self._sc
+        lenp, lenu = Ru.numel(), u.numel()
         st = stream()
-        r_t = empty(Ru.shape)
-        d = empty(u.shape)
+        w = self._work("tomo", 1, u.shape)
+        if "r_t" not in w or w["r_t"].shape != Ru.shape:
+            w["r_t"], w["Rd"] = empty(Ru.shape), empty(Ru.shape)
+        r_t, Rd, d = w["r_t"], w["Rd"], w["d"]
+        sc = w["sc"]
+        w["state"].copy_(w["state0"])             # gamma = 2 (solver.py:184)
+        ct = float(self.coeftomo)
         grad0 = None
-        gamma = 2.0
-        gammas = []
-        beta_dev = torch.zeros(2, dtype=f64, device=device())
-        gam_dev = torch.zeros(_MAXGAM, dtype=f64, device=device())
         for i in range(titer):
+            grad = w["grad"][i % 2]
             # grad = rho*R*(conj(K)(K Ru - xi0)) + tau*J*(J u - xi1)
-            _call("pt_k_tomo_resid", ptr(r_t), ptr(sc[0:]), ptr(K), ptr(Ru), ptr(xi0), lenp, st)
-            grad = self.adj_tomo_batch(r_t, scale=rho)
-            _call("pt_k_reg_grad", ptr(grad), ptr(sc[1:]), ptr(u), ptr(xi1), float(tau), nz, n,
-                  n, st)
-            if i == 0:
-                _call("pt_k_dy_update", ptr(d), ptr(grad), None, 1, lenu, 1, st)
-            else:
-                _call("pt_k_dy_reduce", ptr(sc[2:]), ptr(grad), ptr(grad0), ptr(d), lenu, 1, st)
-                gn, dr, di = self._scalars(5)[2:5]
-                beta = complex(_dai_yuan(np.array([gn]), np.array([complex(dr, di)]))[0])
-                beta_dev.copy_(torch.tensor([beta.real, beta.imag], dtype=f64))
-                _call("pt_k_dy_update", ptr(d), ptr(grad), ptr(beta_dev), 0, lenu, 1, st)
+            _call("pt_k_tomo_resid", ptr(r_t), _dp(sc, 0), ptr(K), ptr(Ru), ptr(xi0), 0.0,
+                  _dp(blk, _RHO), lenp, st)
+            self._tomo.adj_scaled(grad.data_ptr(), r_t.data_ptr(), ct)
+            _call("pt_k_reg_grad", ptr(grad), _dp(sc, 1), ptr(u), ptr(xi1), 0.0, _dp(blk, _TAU),
+                  nz, n, n, st)
+            self._dai_yuan_dev(w, i, grad, grad0, lenu, 1)
             grad0 = grad
-            Rd = self.fwd_tomo_batch(d)
+            self._tomo.fwd_scaled(Rd.data_ptr(), d.data_ptr(), ct)
 
-            def evalf(cand):
-                gam_dev.copy_(torch.from_numpy(cand[0]))
-                _call("pt_k_tomo_minf", ptr(sc[16:]), ptr(K), ptr(Ru), ptr(Rd), ptr(xi0),
-                      ptr(gam_dev), _MAXGAM, lenp, st)
-                _call("pt_k_reg_minf", ptr(sc[24:]), ptr(u), ptr(d), ptr(xi1), ptr(gam_dev),
-                      _MAXGAM, nz, n, n, st)
-                v = self._scalars(32)
-                return (rho * v[16:24] + tau * v[24:32])[None, :]
+            def evalf(cand, state, f1, f2):
+                _call("pt_k_tomo_minf", ptr(f1), ptr(K), ptr(Ru), ptr(Rd), ptr(xi0), ptr(cand),
+                      _MAXGAM, lenp, ptr(state), st)
+                _call("pt_k_reg_minf", ptr(f2), ptr(u), ptr(d), ptr(xi1), ptr(cand), _MAXGAM,
+                      nz, n, n, ptr(state), st)
 
-            gamma = float(self._halving_sweeps([gamma], evalf)[0])
-            gammas.append(gamma)
-            _call("pt_k_lincomb3", ptr(u), 1.0, 0.0, ptr(u), gamma, 0.0, ptr(d), 0.0, 0.0, None,
-                  lenu, st)
-            _call("pt_k_lincomb3", ptr(Ru), 1.0, 0.0, ptr(Ru), gamma, 0.0, ptr(Rd), 0.0, 0.0,
-                  None, lenp, st)
-        self._last_tomo_gammas = gammas
-        return u, Ru
+            # minf = rho*|K R u - xi0|^2 + tau*|J u - xi1|^2 (solver.py:181-182)
+            self._line_search_dev(w, 1, evalf, _dp(blk, _RHO))
+            _call("pt_k_axpy_seg", ptr(u), ptr(w["step"]), ptr(d), lenu, 1, st)
+            _call("pt_k_axpy_seg", ptr(Ru), ptr(w["step"]), ptr(Rd), lenp, 1, st)
+            if hist is not None:
+                hist[i:i + 1].copy_(w["step"][0:1])
 
     # ------------------------------------------------------------------ cg_ptycho
     def cg_ptycho(self, data, init, h, lamd, rho, piter, model):
         """solver.py:206-242 on the chunk set by ``cl_ptycho.setobj`` (data
         already multiplied by coefdata, as the reference passes it)."""
         data = to_dev(data, f32)
-        psi = self._cg_ptycho(self.cl_ptycho, 1, data, init, h, lamd, rho, piter, model,
-                              coefdata=1.0)
+        psi = to_dev(init, c64).clone()
+        self._cg_ptycho_host(self.cl_ptycho, 1, data, psi, h, lamd, rho, piter, model, 1.0)
         return psi
 
     def cg_ptycho_batch(self, data, init, h, lamd, rho, piter, model):
         """solver.py:245-254: all angle chunks in one fused pass with
         per-chunk step lengths and Dai-Yuan coefficients."""
         d = self._dev_data(data)
-        return self._cg_ptycho(self._ptycho, self.nchunk, d, init, h, lamd, rho, piter, model,
-                               coefdata=float(self.coefdata))
-
-    def _cg_ptycho(self, op, nseg, data, init, h, lamd, rho, piter, model, coefdata):
-        mdl = _lib.PT_MODEL[model]
         psi = to_dev(init, c64).clone()
-        h = to_dev(h, c64)
-        lamd = to_dev(lamd, c64)
+        self._cg_ptycho_host(self._ptycho, self.nchunk, d, psi, h, lamd, rho, piter, model,
+                             float(self.coefdata))
+        return psi
+
+    def _cg_ptycho_host(self, op, nseg, data, psi, h, lamd, rho, piter, model, coefdata):
+        blk = self._scalar_block(rho, 0.5)
+        hist = torch.zeros((max(piter, 1), nseg), dtype=f64, device=device())
+        self._cg_ptycho_dev(op, nseg, data, psi, to_dev(h, c64), to_dev(lamd, c64), blk, piter,
+                            model, coefdata, hist)
+        g = hist[:piter].cpu().numpy()
+        self._chunk_gammas = [[float(v) for v in g[:, s]] for s in range(nseg)]
+        self._check_phase(blk)
+
+    def _cg_ptycho_dev(self, op, nseg, data, psi, h, lamd, blk, piter, model, coefdata,
+                       hist=None):
+        """cg_ptycho on psi in place, nseg angle chunks with their own step and
+        Dai-Yuan beta (solver.py:245-254); max|angle psi| goes to blk."""
+        mdl = _lib.PT_MODEL[model]
         st = stream()
         L = psi.numel()
-        sc = torch.zeros(8 * nseg + 8, dtype=f64, device=device())
-        sums = torch.zeros(nseg * _MAXGAM, dtype=f64, device=device())
-        rsums = torch.zeros(nseg * _MAXGAM, dtype=f64, device=device())
-        gam_dev = torch.zeros(nseg * _MAXGAM, dtype=f64, device=device())
-        beta_dev = torch.zeros(2 * nseg, dtype=f64, device=device())
-        step_dev = torch.zeros(2 * nseg, dtype=f64, device=device())
-        grad = empty(psi.shape)
-        grad0 = empty(psi.shape)
-        d = empty(psi.shape)
-        gamma = np.full(nseg, 2.0)
+        w = self._work(("ptycho", id(op)), nseg, psi.shape)
         chunk = op.Ntheta // nseg
         cp, cd = float(self.coefptycho), float(coefdata)
-        gam_hist = [[] for _ in range(nseg)]
+        w["state"].copy_(w["state0"])             # gamma = 2 (solver.py:217)
+        grad0 = None
+        d = w["d"]
         for i in range(piter):
+            grad = w["grad"][i % 2]
             _call("pt_ptycho_grad", op._h, ptr(grad), ptr(psi), ptr(data), ptr(h), ptr(lamd),
-                  float(rho), cp, cd, mdl, st)
-            if i == 0:
-                _call("pt_k_dy_update", ptr(d), ptr(grad), None, 1, L, nseg, st)
-            else:
-                _call("pt_k_dy_reduce", ptr(sc), ptr(grad), ptr(grad0), ptr(d), L, nseg, st)
-                v = sc[:3 * nseg].cpu().numpy().reshape(nseg, 3)
-                beta = _dai_yuan(v[:, 0], v[:, 1] + 1j * v[:, 2])
-                beta_dev.copy_(torch.from_numpy(np.stack([beta.real, beta.imag], 1).ravel()))
-                _call("pt_k_dy_update", ptr(d), ptr(grad), ptr(beta_dev), 0, L, nseg, st)
-            grad, grad0 = grad0, grad
+                  0.0, _dp(blk, _RHO), cp, cd, mdl, st)
+            self._dai_yuan_dev(w, i, grad, grad0, L, nseg)
+            grad0 = grad
 
-            def evalf(cand):
-                gam_dev.copy_(torch.from_numpy(cand.ravel()))
-                _call("pt_ptycho_minf", op._h, ptr(sums), ptr(psi), ptr(d), ptr(data),
-                      ptr(gam_dev), _MAXGAM, chunk, cp, cd, mdl, 0, st)
-                _call("pt_k_rho_minf", ptr(rsums), ptr(h), ptr(psi), ptr(lamd), ptr(d),
-                      float(rho), ptr(gam_dev), _MAXGAM, L, nseg, st)
-                dat = sums.cpu().numpy().reshape(nseg, _MAXGAM)
-                rr = rsums.cpu().numpy().reshape(nseg, _MAXGAM)
-                return dat + rho * rr
+            def evalf(cand, state, f1, f2):
+                _call("pt_ptycho_minf", op._h, ptr(f1), ptr(psi), ptr(d), ptr(data), ptr(cand),
+                      _MAXGAM, chunk, cp, cd, mdl, 0, ptr(state), st)
+                _call("pt_k_rho_minf", ptr(f2), ptr(h), ptr(psi), ptr(lamd), ptr(d), 0.0,
+                      _dp(blk, _RHO), ptr(cand), _MAXGAM, L, nseg, ptr(state), st)
 
-            gamma = self._halving_sweeps(gamma, evalf)
-            for s in range(nseg):
-                gam_hist[s].append(float(gamma[s]))
-            step_dev.copy_(torch.from_numpy(np.stack([gamma, np.zeros(nseg)], 1).ravel()))
-            _call("pt_k_axpy_seg", ptr(psi), ptr(step_dev), ptr(d), L, nseg, st)
-        _call("pt_k_max_angle", ptr(sc), ptr(psi), L, st)
-        mx = float(sc[0].item())
+            # minf = data term + rho*|h - psi + lamd/rho|^2 (solver.py:208-213)
+            self._line_search_dev(w, nseg, evalf, _dp(blk, _ONE))
+            _call("pt_k_axpy_seg", ptr(psi), ptr(w["step"]), ptr(d), L, nseg, st)
+            if hist is not None:
+                hist[i].copy_(w["step"][0::2])
+        _call("pt_k_max_angle", _dp(blk, _MAXANG), ptr(psi), L, st)
+
+    def _check_phase(self, blk):
+        mx = float(blk[_MAXANG].item())
         if mx > 3.14:
             print('possible phase wrap, max computed angle', mx)
-        self._chunk_gammas = gam_hist
-        return psi
 
     # ------------------------------------------------------------------ penalties / Lagrangian
     def update_penalty(self, psi, h, h0, phi, e, e0, rho, tau):
         """solver.py:267-282."""
+        blk = self._scalar_block(rho, tau)
+        psi, h, h0 = to_dev(psi, c64), to_dev(h, c64), to_dev(h0, c64)
+        phi, e, e0 = to_dev(phi, c64), to_dev(e, c64), to_dev(e0, c64)
+        self._penalty_dev(psi, h, h0, phi, e, e0, blk)
+        v = blk[_RHO:_TAU + 1].cpu().numpy()
+        return float(v[0]), float(v[1])
+
+    def _penalty_dev(self, psi, h, h0, phi, e, e0, blk):
         st = stream()
-        _call("pt_k_norm2_lin", ptr(self._sc[0:]), ptr(psi), ptr(h), 1.0, 0.0, -1.0, 0.0,
-              psi.numel(), 1, st)
-        _call("pt_k_norm2_lin", ptr(self._sc[1:]), ptr(h), ptr(h0), rho, 0.0, -rho, 0.0,
-              h.numel(), 1, st)
-        _call("pt_k_norm2_lin", ptr(self._sc[2:]), ptr(phi), ptr(e), 1.0, 0.0, -1.0, 0.0,
-              phi.numel(), 1, st)
-        _call("pt_k_norm2_lin", ptr(self._sc[3:]), ptr(e), ptr(e0), tau, 0.0, -tau, 0.0,
-              e.numel(), 1, st)
-        r, s, r2, s2 = self._scalars(4)
-        if r > 10 * s:
-            rho *= 2
-        elif s > 10 * r:
-            rho *= 0.5
-        if r2 > 10 * s2:
-            tau *= 2
-        elif s2 > 10 * r2:
-            tau *= 0.5
-        return rho, tau
+        for k, (a, b) in enumerate(((psi, h), (h, h0), (phi, e), (e, e0))):
+            _call("pt_k_norm2_lin", _dp(blk, _NORMS + k), ptr(a), ptr(b), 1.0, 0.0, -1.0, 0.0,
+                  a.numel(), 1, st)
+        _call("pt_k_penalty", _dp(blk, _RHO), _dp(blk, _NORMS), st)
 
     def take_lagr(self, psi, phi, data, h, e, lamd, mu, alpha, rho, tau, model):
         """solver.py:285-305 (lagr[6] = sum(lagr[0:5]) as in the reference)."""
         st = stream()
         dd = self._dev_data(data)
+        psi, phi, h, e = to_dev(psi, c64), to_dev(phi, c64), to_dev(h, c64), to_dev(e, c64)
+        lamd, mu = to_dev(lamd, c64), to_dev(mu, c64)
         lagr = np.zeros(7, dtype=np.float32)
         sums = torch.zeros(self.nchunk, dtype=f64, device=device())
         gz = torch.zeros(self.nchunk, dtype=f64, device=device())
         _call("pt_ptycho_minf", self._ptycho._h, ptr(sums), ptr(psi), None, ptr(dd), ptr(gz), 1,
               self.ptheta, float(self.coefptycho), float(self.coefdata), _lib.PT_MODEL[model],
-              1, st)
+              1, None, st)
         plane = self.nz * self.n * self.n
         _call("pt_k_tv_norm", ptr(self._sc[0:]), ptr(phi), plane, st)
         _call("pt_k_dot_diff", ptr(self._sc[1:]), ptr(lamd), ptr(h), ptr(psi), h.numel(), st)
@@ -499,56 +495,107 @@ class Solver(object):
 
     # ------------------------------------------------------------------ ADMM
     def admm(self, data, h, e, psi, phi, lamd, mu, u, alpha, piter, titer, niter, model,
-             lagr_every=10, verbose=True, rho=0.5, tau=0.5, m0=0, Ru=None, callback=None):
+             lagr_every=10, verbose=True, rho=0.5, tau=0.5, m0=0, Ru=None, callback=None,
+             graph=False):
         """solver.py:308-336.  Returns (u, psi) like the reference.
+
+        Each iteration is enqueued on the device without host round trips:
+        step lengths, Dai-Yuan coefficients, the takexi phase and the penalty
+        update are computed by device kernels, and the state is updated in
+        place in buffers owned by this call (the caller's arrays are not
+        modified).  One synchronisation per iteration reads the steps and
+        penalties into ``self.history``.  With ``graph=True`` the steady-state
+        iteration (m >= 2, piter CG iterations) is captured once as a HIP
+        graph and replayed.
 
         Beyond the reference: the full final state (h, e, psi, phi, lamd, mu,
         u, rho, tau, Ru) is kept in ``self.state``; ``rho``, ``tau``, ``m0``
         (index of the first iteration, which sets the piter+32 schedule of
         the first two iterations and the Lagrangian cadence) and ``Ru``
         (R u of the initial u) resume a run exactly where ``self.state`` left
-        it.  ``callback(m, solver_locals)`` is called after every iteration.
+        it.  ``callback(m, state)`` is called after every iteration with the
+        live state buffers.
         """
         data = self._dev_data(data)
-        h, e, psi = to_dev(h, c64), to_dev(e, c64), to_dev(psi, c64)
-        phi, lamd, mu, u = to_dev(phi, c64), to_dev(lamd, c64), to_dev(mu, c64), to_dev(u, c64)
-        st = stream()
+        S = {k: to_dev(v, c64).clone() for k, v in
+             dict(h=h, e=e, psi=psi, phi=phi, lamd=lamd, mu=mu, u=u).items()}
+        S["Ru"] = self.fwd_tomo_batch(S["u"]) if Ru is None else to_dev(Ru, c64).clone()
+        S["hn"], S["en"] = empty(S["h"].shape), empty(S["e"].shape)
+        S["xi0"], S["K"], S["xi1"] = empty(S["psi"].shape), empty(S["psi"].shape), \
+            empty(S["phi"].shape)
+        blk = self._scalar_block(rho, tau)
+        pmax = piter + 32
+        H = {"pg": torch.zeros((pmax, self.nchunk), dtype=f64, device=device()),
+             "tg": torch.zeros(max(titer, 1), dtype=f64, device=device()),
+             "rt": torch.zeros(2, dtype=f64, device=device())}
         self.history = []
+        graph_obj = None
         for m in range(m0, m0 + niter):
-            h0, e0 = h, e
-            psi = self.cg_ptycho_batch(data, psi, h, lamd, rho, piter + (m < 2) * 32, model)
-            xi0, xi1, K, pshift = self.takexi(psi, phi, lamd, mu, rho, tau)
-            u, Ru = self._cg_tomo(xi0, xi1, K, u, rho, tau, titer, Ru=Ru)
-            phi = self.solve_reg(u, mu, tau, alpha)
-            # h = exptomo(fwd_tomo_batch(u)) * exp(i pshift), R u from the CG recursion
-            h = empty(psi.shape)
-            c = self.voxelsize * self.wavenumber() / float(self.coeftomo)
-            _call("pt_k_exptomo", ptr(h), ptr(Ru), c, float(pshift), h.numel(), st)
-            e = self.fwd_reg(u)
-            lamd_n = empty(lamd.shape)
-            _call("pt_k_lincomb3", ptr(lamd_n), 1.0, 0.0, ptr(lamd), rho, 0.0, ptr(h), -rho, 0.0,
-                  ptr(psi), lamd.numel(), st)
-            mu_n = empty(mu.shape)
-            _call("pt_k_lincomb3", ptr(mu_n), 1.0, 0.0, ptr(mu), tau, 0.0, ptr(e), -tau, 0.0,
-                  ptr(phi), mu.numel(), st)
-            lamd, mu = lamd_n, mu_n
-            rec = {"m": m, "rho_used": rho, "tau_used": tau, "pshift": float(pshift),
-                   "ptycho_gammas": self._chunk_gammas, "tomo_gammas": self._last_tomo_gammas}
-            rho, tau = self.update_penalty(psi, h, h0, phi, e, e0, rho, tau)
-            rec["rho"], rec["tau"] = rho, tau
+            pit = piter + (m < 2) * 32
+            args = (S, blk, H, data, pit, titer, alpha, model)
+            if graph and m >= 2 and m > m0:
+                if graph_obj is None:
+                    torch.cuda.synchronize()
+                    graph_obj = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(graph_obj):
+                        self._admm_iter(*args)
+                graph_obj.replay()
+            else:
+                self._admm_iter(*args)
+            # one synchronisation per iteration: steps, penalties, phase
+            v = torch.cat([H["rt"], blk[:_NSC], H["tg"], H["pg"][:pit].reshape(-1)]).cpu().numpy()
+            rho_used, tau_used = float(v[0]), float(v[1])
+            b = v[2:2 + _NSC]
+            rho, tau = float(b[_RHO]), float(b[_TAU])
+            tg = v[2 + _NSC:2 + _NSC + H["tg"].numel()][:titer]
+            pg = v[2 + _NSC + H["tg"].numel():].reshape(pit, self.nchunk)
+            if b[_MAXANG] > 3.14:
+                print('possible phase wrap, max computed angle', float(b[_MAXANG]))
+            rec = {"m": m, "rho_used": rho_used, "tau_used": tau_used, "pshift": float(b[_PRM]),
+                   "ptycho_gammas": [[float(x) for x in pg[:, s]] for s in range(self.nchunk)],
+                   "tomo_gammas": [float(x) for x in tg], "rho": rho, "tau": tau}
+            self._chunk_gammas, self._last_tomo_gammas = rec["ptycho_gammas"], rec["tomo_gammas"]
             if lagr_every and np.mod(m, lagr_every) == 0:
-                lagr = self.take_lagr(psi, phi, data, h, e, lamd, mu, alpha, rho, tau, model)
-                rec["lagr"] = [float(v) for v in lagr]
+                lagr = self.take_lagr(S["psi"], S["phi"], data, S["h"], S["e"], S["lamd"], S["mu"],
+                                      alpha, rho, tau, model)
+                rec["lagr"] = [float(x) for x in lagr]
                 if verbose:
                     print("%d/%d) rho=%.2e, tau=%.2e, Lagrangian terms:  %.2e %.2e %.2e %.2e %.2e %.2e, Sum: %.2e" %
                           (m, niter, rho, tau, *lagr))
             self.history.append(rec)
             if callback is not None:
-                callback(m, dict(h=h, e=e, psi=psi, phi=phi, lamd=lamd, mu=mu, u=u, rho=rho,
-                                 tau=tau, Ru=Ru))
-        self.state = dict(h=h, e=e, psi=psi, phi=phi, lamd=lamd, mu=mu, u=u, rho=rho, tau=tau,
-                          Ru=Ru, m=m0 + niter)
-        return u, psi
+                callback(m, dict(h=S["h"], e=S["e"], psi=S["psi"], phi=S["phi"], lamd=S["lamd"],
+                                 mu=S["mu"], u=S["u"], rho=rho, tau=tau, Ru=S["Ru"]))
+        rho, tau = (float(x) for x in blk[_RHO:_TAU + 1].cpu().numpy())
+        self.state = dict(h=S["h"], e=S["e"], psi=S["psi"], phi=S["phi"], lamd=S["lamd"],
+                          mu=S["mu"], u=S["u"], rho=rho, tau=tau, Ru=S["Ru"], m=m0 + niter)
+        return S["u"], S["psi"]
+
+    def _admm_iter(self, S, blk, H, data, pit, titer, alpha, model):
+        """One ADMM iteration (solver.py:312-330) on the state S in place, all
+        scalars on the device (blk), steps recorded into H."""
+        st = stream()
+        H["rt"].copy_(blk[_RHO:_TAU + 1])
+        self._cg_ptycho_dev(self._ptycho, self.nchunk, data, S["psi"], S["h"], S["lamd"], blk,
+                            pit, model, float(self.coefdata), H["pg"])
+        self._takexi_dev(S["psi"], S["phi"], S["lamd"], S["mu"], blk, S["xi0"], S["K"], S["xi1"])
+        self._cg_tomo_dev(S["xi0"], S["xi1"], S["K"], S["u"], S["Ru"], blk, titer, H["tg"])
+        u = S["u"]
+        _call("pt_k_reg_solve", ptr(S["phi"]), ptr(u), ptr(S["mu"]), 0.0, _dp(blk, _TAU),
+              float(alpha), *u.shape, st)
+        # h = exptomo(fwd_tomo_batch(u)) * exp(i pshift), R u from the CG recursion
+        c = self.voxelsize * self.wavenumber() / float(self.coeftomo)
+        _call("pt_k_exptomo", ptr(S["hn"]), ptr(S["Ru"]), c, 0.0, _dp(blk, _PRM),
+              S["hn"].numel(), st)
+        _call("pt_k_reg_fwd", ptr(S["en"]), ptr(u), *u.shape, st)
+        # lamd += rho (h - psi), mu += tau (e - phi)  (solver.py:326-327)
+        _call("pt_k_dual", ptr(S["lamd"]), ptr(S["lamd"]), ptr(S["hn"]), ptr(S["psi"]), 0.0,
+              _dp(blk, _RHO), 0, S["lamd"].numel(), st)
+        _call("pt_k_dual", ptr(S["mu"]), ptr(S["mu"]), ptr(S["en"]), ptr(S["phi"]), 0.0,
+              _dp(blk, _TAU), 0, S["mu"].numel(), st)
+        self._penalty_dev(S["psi"], S["hn"], S["h"], S["phi"], S["en"], S["e"], blk)
+        S["h"].copy_(S["hn"])
+        S["e"].copy_(S["en"])
 
     def admm_resume(self, data, alpha, piter, titer, niter, model, **kw):
         """Continue from ``self.state`` (checkpoint/resume of the full state)."""
@@ -556,3 +603,8 @@ class Solver(object):
         return self.admm(data, s["h"], s["e"], s["psi"], s["phi"], s["lamd"], s["mu"], s["u"],
                          alpha, piter, titer, niter, model, rho=s["rho"], tau=s["tau"],
                          m0=s["m"], Ru=s["Ru"], **kw)
+
+
+def _dp(t, k=0):
+    """device address of element k of a float64 tensor"""
+    return ctypes.c_void_p(t.data_ptr() + 8 * k)

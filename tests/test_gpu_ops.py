@@ -182,7 +182,7 @@ def test_solver_fbp_inverts_radon():
     theta = np.linspace(0, np.pi, ntheta, endpoint=False).astype(np.float32)
     prb = objects.gaussian_probe(16, 4.0, 0.5)
     scan = np.zeros((2, ntheta, 4), np.float32)
-    slv = Solver(prb, scan, theta, 16, 1e-6, 8.8, ntheta, nz, n, ntheta, nz)
+    slv = Solver(prb, scan, theta, [16, 16], 1e-6, 8.8, ntheta, nz, n, ntheta, nz)
     y, x = np.mgrid[:n, :n] - n / 2 + 0.5
     u = np.exp(-(x ** 2 + y ** 2) / (2 * (n / 10) ** 2)).astype(np.complex64)
     u = np.repeat(u[None], nz, 0) * (1 + 0.1j)
