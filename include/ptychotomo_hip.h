@@ -269,6 +269,65 @@ int pt_k_ls_select(double* state, double* cand, double* step, const double* f1, 
  * place from norms = {||psi-h||^2, ||h-h0||^2, ||phi-e||^2, ||e-e0||^2}. */
 int pt_k_penalty(double* rho_tau, const double* norms, pt_stream stream);
 
+/* ------------------------------------------------------- warm-start 3D U-Net
+ * No counterpart in the reference: a learned initial guess for ADMM.
+ * Activations are bf16 channels-last [D][H][W][C] (raw 16-bit storage);
+ * conv weights bf16 [Cout][Kp] with k = tap*Cin + ci, tap = 9 dz + 3 dy + dx
+ * for offsets (dz-1, dy-1, dx-1), zero-padded to Kp = ceil16(27 Cin);
+ * master weights FP32 [Cout][27][Cin].  Convolutions run on the matrix
+ * cores (bf16 in, FP32 accumulate); reductions are deterministic. */
+/* y = [relu](bias + conv3x3x3(x, w)), zero padding; Cout in {16, 32, 64};
+ * bias may be NULL. */
+int pt_nn_conv3d(void* y, const void* x, const void* w, const float* bias, int D, int H, int W,
+                 int Cin, int Cout, int relu, pt_stream stream);
+/* weights of the input gradient: wt[ci][t*Cout + co] = w[co][(26-t)*Cin + ci]
+ * (dx = pt_nn_conv3d(dy, wt) with Cin, Cout swapped, no bias). */
+int pt_nn_conv3d_wflip(void* wt, const void* w, int Cin, int Cout, pt_stream stream);
+/* weight and bias gradients of pt_nn_conv3d for the pre-activation gradient
+ * dy: dw [Cout][27][Cin] FP32, dbias [Cout] (may be NULL); part (FP32,
+ * part_cap floats) and bpart (256*Cout doubles) are workspaces. */
+int pt_nn_conv3d_wgrad(float* dw, float* dbias, float* part, long part_cap, double* bpart,
+                       const void* dy, const void* x, int D, int H, int W, int Cin, int Cout,
+                       pt_stream stream);
+/* dpre = y > 0 ? dy : 0 (bf16, n elements) */
+int pt_nn_relu_bwd(void* dpre, const void* dy, const void* y, long n, pt_stream stream);
+/* 2x average pooling of [D][H][W][C] and its gradient (dx = dy(parent)/8,
+ * plus `add` if not NULL). */
+int pt_nn_pool(void* y, const void* x, int D, int H, int W, int C, pt_stream stream);
+int pt_nn_pool_bwd(void* dx, const void* dy, const void* add, int D, int H, int W, int C,
+                   pt_stream stream);
+/* nearest 2x upsampling of [d][h][w][C] into channels [c0, c0+C) of a
+ * Ct-channel [2d][2h][2w] tensor, and its gradient (sum of the 8 children). */
+int pt_nn_up(void* y, const void* x, int d, int h, int w, int C, int Ct, int c0, pt_stream stream);
+int pt_nn_up_bwd(void* dx, const void* dy, int d, int h, int w, int C, int Ct, int c0,
+                 pt_stream stream);
+/* dst[v][d0 + c] = src[v][s0 + c], c < C (concatenation) */
+int pt_nn_chan_copy(void* dst, int Cd, int d0, const void* src, int Cs, int s0, int C, long V,
+                    pt_stream stream);
+/* residual 1x1x1 head (FP32 out): out[v][k] = inp[v][k] + b[k] + sum_c w[k][c] x[v][c] */
+int pt_nn_head(float* out, const void* x, const float* w, const float* b, const float* inp,
+               long V, int C, int K, pt_stream stream);
+/* head gradient: dx (bf16) and dwb[k][c] (c < C: weights, c = C: bias),
+ * part: part_cap doubles of workspace.  Built for C = 16, K = 2. */
+int pt_nn_head_bwd(void* dx, float* dwb, double* part, long part_cap, const float* dout,
+                   const void* x, const float* w, long V, int C, int K, pt_stream stream);
+/* MSE: part[b] = block partial of sum (pred-target)^2 (FP64, nblk blocks);
+ * grad = 2 (pred - target)/n if not NULL. */
+int pt_nn_mse(double* part, int nblk, float* grad, const float* pred, const float* target,
+              long n, pt_stream stream);
+/* Adam step t (bias-corrected) on FP32 master weights; wb, if not NULL,
+ * receives the bf16 conv copy (master index co*K + k -> co*Kp + k). */
+int pt_nn_adam(float* wm, void* wb, const float* g, float* m, float* v, long n, int K, int Kp,
+               float lr, float b1, float b2, float eps, int t, pt_stream stream);
+/* bf16 conv weights from FP32 masters */
+int pt_nn_pack_w(void* wb, const float* wm, int Cout, int Cin, pt_stream stream);
+/* FP32 <-> bf16 (round to nearest even) */
+int pt_nn_to_bf16(void* y, const float* x, long n, pt_stream stream);
+int pt_nn_to_f32(float* y, const void* x, long n, pt_stream stream);
+/* c (16x16 FP32, row-major) = a * b for bf16 16x16 row-major a, b: one
+ * matrix-core instruction (operand-layout check). */
+int pt_nn_mfma_tile(float* c, const void* a, const void* b, pt_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,25 @@ _SIGS = {
     "pt_k_ls_begin": (c_int, [vp, vp, c_int, c_int, vp]),
     "pt_k_ls_select": (c_int, [vp, vp, vp, vp, vp, vp, c_int, c_int, c_int, vp]),
     "pt_k_penalty": (c_int, [vp, vp, vp]),
+    "pt_nn_conv3d": (c_int, [vp, vp, vp, vp, c_int, c_int, c_int, c_int, c_int, c_int, vp]),
+    "pt_nn_conv3d_wflip": (c_int, [vp, vp, c_int, c_int, vp]),
+    "pt_nn_conv3d_wgrad": (c_int, [vp, vp, vp, c_long, vp, vp, vp, c_int, c_int, c_int, c_int,
+                                   c_int, vp]),
+    "pt_nn_relu_bwd": (c_int, [vp, vp, vp, c_long, vp]),
+    "pt_nn_pool": (c_int, [vp, vp, c_int, c_int, c_int, c_int, vp]),
+    "pt_nn_pool_bwd": (c_int, [vp, vp, vp, c_int, c_int, c_int, c_int, vp]),
+    "pt_nn_up": (c_int, [vp, vp, c_int, c_int, c_int, c_int, c_int, c_int, vp]),
+    "pt_nn_up_bwd": (c_int, [vp, vp, c_int, c_int, c_int, c_int, c_int, c_int, vp]),
+    "pt_nn_chan_copy": (c_int, [vp, c_int, c_int, vp, c_int, c_int, c_int, c_long, vp]),
+    "pt_nn_head": (c_int, [vp, vp, vp, vp, vp, c_long, c_int, c_int, vp]),
+    "pt_nn_head_bwd": (c_int, [vp, vp, vp, c_long, vp, vp, vp, c_long, c_int, c_int, vp]),
+    "pt_nn_mse": (c_int, [vp, c_int, vp, vp, vp, c_long, vp]),
+    "pt_nn_adam": (c_int, [vp, vp, vp, vp, vp, c_long, c_int, c_int, c_float, c_float, c_float,
+                           c_float, c_int, vp]),
+    "pt_nn_pack_w": (c_int, [vp, vp, c_int, c_int, vp]),
+    "pt_nn_to_bf16": (c_int, [vp, vp, c_long, vp]),
+    "pt_nn_to_f32": (c_int, [vp, vp, c_long, vp]),
+    "pt_nn_mfma_tile": (c_int, [vp, vp, vp, vp]),
 }
 
 EXPORTED = sorted(_SIGS)

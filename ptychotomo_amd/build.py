@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libptychotomo_hip.so")
 ARCH = os.environ.get("PT_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["vec.hip", "ptycho.hip", "tomo.hip"]
+SOURCES = ["vec.hip", "ptycho.hip", "tomo.hip", "unet.hip"]
 
 
 def hipcc():

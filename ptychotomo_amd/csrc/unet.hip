@@ -1,0 +1,700 @@
+// Warm-start 3D U-Net kernels for gfx950: bf16 activations and weights, FP32
+// accumulation on the matrix cores (v_mfma_f32_16x16x16_bf16).
+//
+// No counterpart in the reference: the network seeds ADMM (an addition on
+// top of the ADMM path).  Layouts: activations channels-last [D][H][W][C]
+// bf16; conv weights [Cout][Kp] bf16 with k = tap*Cin + ci (tap = 9 dz +
+// 3 dy + dx for offsets (dz-1, dy-1, dx-1)) zero-padded to Kp, a multiple of
+// 16; FP32 master weights [Cout][27][Cin] are kept by the optimiser.
+//
+// Convolution = implicit GEMM  Y^T (Cout x V) = Wk (Cout x Kp) * Xcol (Kp x V):
+// a wavefront computes Cout x (16*NT) outputs; per 16-wide k step it loads
+// one 4 x bf16 A fragment per 16 output channels (weights) and one B fragment
+// per 16 voxels (4 consecutive input channels of one tap, zero outside the
+// volume), and issues MT*NT MFMAs.  MFMA 16x16x16 operand layout (lane l):
+// A[i][k]: i = l%16, k = 4*(l/16)+e; B[k][j]: j = l%16, k = 4*(l/16)+e;
+// D[i][j]: j = l%16, i = 4*(l/16)+e (e = element 0..3 of the lane's vector).
+//
+// Weight gradient = GEMM over voxels (split-K): each wavefront owns a
+// 16 x 16 (Cout, tap*Cin) tile for one voxel range and writes an FP32
+// partial; a second pass sums the partials in a fixed order (deterministic).
+#include <cmath>
+
+#include "common.hpp"
+
+namespace pt {
+namespace {
+
+using bf16 = unsigned short;
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((unsigned)h << 16); }
+// float -> bf16, round to nearest even (NaN kept quiet)
+__device__ __forceinline__ bf16 f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (bf16)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16)(u >> 16);
+}
+
+// sum of one double over a 256-thread block (fixed order), result in thread 0
+__device__ __forceinline__ double block_sum1(double v, double* sm) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sm[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += sm[q];
+  __syncthreads();
+  return s;
+}
+
+__device__ __forceinline__ void tap_off(int t, int& oz, int& oy, int& ox) {
+  oz = t / 9 - 1;
+  oy = (t / 3) % 3 - 1;
+  ox = t % 3 - 1;
+}
+
+// ------------------------------------------------------------------ conv forward
+// MT = Cout/16; NT voxel tiles of 16 per wavefront; C16: Cin % 16 == 0 (one tap
+// per k step, vector loads), else the generic per-element gather.
+template <int MT, int NT, bool C16>
+__global__ __launch_bounds__(256) void k_conv3d(bf16* __restrict__ y, const bf16* __restrict__ x,
+                                                const bf16* __restrict__ w,
+                                                const float* __restrict__ bias, int D, int H,
+                                                int W, int Cin, int Kp, int relu) {
+  const long V = (long)D * H * W;
+  const long v0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 * NT;
+  if (v0 >= V) return;
+  const int l = threadIdx.x & 63, col = l & 15, kq = l >> 4;
+  const int K = 27 * Cin;
+  int vz[NT], vy[NT], vx[NT];
+  bool vin[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    const long v = v0 + nt * 16 + col;
+    vin[nt] = v < V;
+    const long vv = vin[nt] ? v : 0;
+    vx[nt] = (int)(vv % W);
+    vy[nt] = (int)((vv / W) % H);
+    vz[nt] = (int)(vv / ((long)W * H));
+  }
+  f4v acc[MT][NT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f4v{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < Kp; k0 += 16) {
+    s4v a[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+      a[mt] = *reinterpret_cast<const s4v*>(w + (long)(mt * 16 + col) * Kp + k0 + 4 * kq);
+    s4v b[NT];
+    if constexpr (C16) {
+      const int t = k0 / Cin, ci = k0 - t * Cin + 4 * kq;
+      int oz, oy, ox;
+      tap_off(t, oz, oy, ox);
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const int z = vz[nt] + oz, yy = vy[nt] + oy, xx = vx[nt] + ox;
+        const bool in = vin[nt] && z >= 0 && z < D && yy >= 0 && yy < H && xx >= 0 && xx < W;
+        b[nt] = in ? *reinterpret_cast<const s4v*>(x + (((long)z * H + yy) * W + xx) * Cin + ci)
+                   : s4v{0, 0, 0, 0};
+      }
+    } else {
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        short e4[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int k = k0 + 4 * kq + e;
+          short val = 0;
+          if (k < K && vin[nt]) {
+            const int t = k / Cin, ci = k - t * Cin;
+            int oz, oy, ox;
+            tap_off(t, oz, oy, ox);
+            const int z = vz[nt] + oz, yy = vy[nt] + oy, xx = vx[nt] + ox;
+            if (z >= 0 && z < D && yy >= 0 && yy < H && xx >= 0 && xx < W)
+              val = (short)x[(((long)z * H + yy) * W + xx) * Cin + ci];
+          }
+          e4[e] = val;
+        }
+        b[nt] = s4v{e4[0], e4[1], e4[2], e4[3]};
+      }
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+  }
+  const int Cout = MT * 16;
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    if (!vin[nt]) continue;
+    const long v = v0 + nt * 16 + col;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int c0 = mt * 16 + 4 * kq;
+      bf16 o[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        float r = acc[mt][nt][e] + (bias ? bias[c0 + e] : 0.f);
+        if (relu) r = fmaxf(r, 0.f);
+        o[e] = f2bf(r);
+      }
+      const unsigned lo = (unsigned)o[0] | ((unsigned)o[1] << 16);
+      const unsigned hi = (unsigned)o[2] | ((unsigned)o[3] << 16);
+      *reinterpret_cast<uint2*>(y + v * Cout + c0) = make_uint2(lo, hi);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ weight gradient
+// part[s][co][n] = sum over the voxels of split s of dy[v][co] * x[v + off_t][ci],
+// n = t*Cin + ci (n < 27*Cin; padded columns are 0).
+__global__ __launch_bounds__(256) void k_conv3d_wgrad(float* __restrict__ part,
+                                                      const bf16* __restrict__ dy,
+                                                      const bf16* __restrict__ x, int D, int H,
+                                                      int W, int Cin, int Cout, int Np,
+                                                      long chunk) {
+  const long V = (long)D * H * W;
+  const int ntn = Np / 16, ntm = Cout / 16;
+  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= ntm * ntn) return;
+  const int mt = tile / ntn, ntile = tile - mt * ntn;
+  const int split = blockIdx.y;
+  const int l = threadIdx.x & 63, col = l & 15, kq = l >> 4;
+  const int K = 27 * Cin;
+  const int n = ntile * 16 + col;             // this lane's B column
+  const bool nvalid = n < K;
+  const int t = nvalid ? n / Cin : 0, ci = nvalid ? n - t * Cin : 0;
+  int oz, oy, ox;
+  tap_off(t, oz, oy, ox);
+  const int co = mt * 16 + col;               // this lane's A row
+  f4v acc = {0.f, 0.f, 0.f, 0.f};
+  const long vb0 = (long)split * chunk, vb1 = min(V, vb0 + chunk);
+  for (long vb = vb0; vb < vb1; vb += 16) {
+    short ea[4], eb[4];
+    long v = vb + 4 * kq;
+    int x0 = (int)(v % W), y0 = (int)((v / W) % H), z0 = (int)(v / ((long)W * H));
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      short av = 0, bv = 0;
+      if (v < vb1) {
+        av = (short)dy[v * Cout + co];
+        const int z = z0 + oz, yy = y0 + oy, xx = x0 + ox;
+        if (nvalid && z >= 0 && z < D && yy >= 0 && yy < H && xx >= 0 && xx < W)
+          bv = (short)x[(((long)z * H + yy) * W + xx) * Cin + ci];
+      }
+      ea[e] = av;
+      eb[e] = bv;
+      ++v;
+      if (++x0 == W) { x0 = 0; if (++y0 == H) { y0 = 0; ++z0; } }
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(s4v{ea[0], ea[1], ea[2], ea[3]},
+                                                   s4v{eb[0], eb[1], eb[2], eb[3]}, acc, 0, 0, 0);
+  }
+  // D rows = output channels mt*16 + 4*kq + e, column = n
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int c = mt * 16 + 4 * kq + e;
+    part[((long)split * Cout + c) * Np + ntile * 16 + col] = acc[e];
+  }
+}
+
+// dw_master[co][k] (k < 27*Cin) = sum_s part[s][co][k] in split order (FP64)
+__global__ void k_wgrad_finish(float* __restrict__ dw, const float* __restrict__ part, int nsplit,
+                               int Cout, int Np, int K) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)Cout * K) return;
+  const int co = (int)(i / K), k = (int)(i - (long)co * K);
+  double s = 0.0;
+  for (int sp = 0; sp < nsplit; ++sp) s += part[((long)sp * Cout + co) * Np + k];
+  dw[i] = (float)s;
+}
+
+// per-channel sums over voxels of a channels-last tensor (FP32 or bf16 input):
+// part[b][c]; finish sums the blocks in order
+template <typename T>
+__device__ __forceinline__ float ld_f(const T* p, long i);
+template <>
+__device__ __forceinline__ float ld_f<float>(const float* p, long i) { return p[i]; }
+template <>
+__device__ __forceinline__ float ld_f<bf16>(const bf16* p, long i) { return bf2f(p[i]); }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_chan_sum(double* __restrict__ part, const T* __restrict__ g,
+                                                  long V, int C) {
+  // thread = (channel, voxel lane): C <= 64, 256/C voxel lanes
+  const int c = threadIdx.x % C, lanes = 256 / C, vl = threadIdx.x / C;
+  if (vl >= lanes) return;
+  double s = 0.0;
+  for (long v = (long)blockIdx.x * lanes + vl; v < V; v += (long)gridDim.x * lanes)
+    s += (double)ld_f<T>(g, v * C + c);
+  __shared__ double sm[256];
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  if (vl == 0) {
+    double t = 0.0;
+    for (int q = 0; q < lanes; ++q) t += sm[q * C + c];
+    part[(long)blockIdx.x * C + c] = t;
+  }
+}
+__global__ void k_chan_finish(float* __restrict__ out, const double* __restrict__ part, int nblk,
+                              int C) {
+  const int c = threadIdx.x;
+  if (c >= C) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[(long)b * C + c];
+  out[c] = (float)s;
+}
+
+// ------------------------------------------------------------------ element-wise layers
+// flipped / transposed weights for the input gradient:
+// wt[ci][t*Cout + co] = w[co][(26 - t)*Cin + ci]
+__global__ void k_wflip(bf16* __restrict__ wt, const bf16* __restrict__ w, int Cin, int Cout,
+                        int Kp, int Kpt) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)Cin * Kpt) return;
+  const int ci = (int)(i / Kpt), k = (int)(i - (long)ci * Kpt);
+  bf16 v = 0;
+  if (k < 27 * Cout) {
+    const int t = k / Cout, co = k - t * Cout;
+    v = w[(long)co * Kp + (26 - t) * Cin + ci];
+  }
+  wt[i] = v;
+}
+
+__global__ void k_relu_bwd(bf16* __restrict__ dpre, const bf16* __restrict__ dy,
+                           const bf16* __restrict__ y, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dpre[i] = bf2f(y[i]) > 0.f ? dy[i] : (bf16)0;
+}
+
+// 2x average pooling, channels-last
+__global__ void k_pool(bf16* __restrict__ y, const bf16* __restrict__ x, int D, int H, int W,
+                       int C) {
+  const int d = D / 2, h = H / 2, wd = W / 2;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)d * h * wd * C) return;
+  const int c = (int)(i % C);
+  long r = i / C;
+  const int xo = (int)(r % wd); r /= wd;
+  const int yo = (int)(r % h);
+  const int zo = (int)(r / h);
+  float s = 0.f;
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dyy = 0; dyy < 2; ++dyy)
+      for (int dx = 0; dx < 2; ++dx)
+        s += bf2f(x[((((long)(2 * zo + dz) * H) + 2 * yo + dyy) * W + 2 * xo + dx) * C + c]);
+  y[i] = f2bf(s * 0.125f);
+}
+// gradient of pooling: dx[v] = dy[parent]/8, added to acc (FP32 sum, bf16 out)
+__global__ void k_pool_bwd(bf16* __restrict__ dx, const bf16* __restrict__ dy,
+                           const bf16* __restrict__ add, int D, int H, int W, int C) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)D * H * W * C) return;
+  const int c = (int)(i % C);
+  long r = i / C;
+  const int xx = (int)(r % W); r /= W;
+  const int yy = (int)(r % H);
+  const int zz = (int)(r / H);
+  const long p = ((((long)(zz / 2) * (H / 2)) + yy / 2) * (W / 2) + xx / 2) * C + c;
+  float v = bf2f(dy[p]) * 0.125f;
+  if (add) v += bf2f(add[i]);
+  dx[i] = f2bf(v);
+}
+// nearest 2x upsampling written into channels [c0, c0+C) of a Ct-channel
+// tensor (the concatenation)
+__global__ void k_up(bf16* __restrict__ y, const bf16* __restrict__ x, int d, int h, int w, int C,
+                     int Ct, int c0) {
+  const int D = 2 * d, H = 2 * h, W = 2 * w;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)D * H * W * C) return;
+  const int c = (int)(i % C);
+  long r = i / C;
+  const int xx = (int)(r % W); r /= W;
+  const int yy = (int)(r % H);
+  const int zz = (int)(r / H);
+  y[(((long)zz * H + yy) * W + xx) * Ct + c0 + c] =
+      x[((((long)(zz / 2) * h) + yy / 2) * w + xx / 2) * C + c];
+}
+// gradient of upsampling from channels [c0, c0+C) of a Ct-channel gradient
+__global__ void k_up_bwd(bf16* __restrict__ dx, const bf16* __restrict__ dy, int d, int h, int w,
+                         int C, int Ct, int c0) {
+  const int H = 2 * h, W = 2 * w;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)d * h * w * C) return;
+  const int c = (int)(i % C);
+  long r = i / C;
+  const int xo = (int)(r % w); r /= w;
+  const int yo = (int)(r % h);
+  const int zo = (int)(r / h);
+  float s = 0.f;
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dyy = 0; dyy < 2; ++dyy)
+      for (int dxx = 0; dxx < 2; ++dxx)
+        s += bf2f(dy[((((long)(2 * zo + dz) * H) + 2 * yo + dyy) * W + 2 * xo + dxx) * Ct + c0 + c]);
+  dx[i] = f2bf(s);
+}
+// channel-range copy (concatenation part and its gradient)
+__global__ void k_chan_copy(bf16* __restrict__ dst, int Cd, int d0, const bf16* __restrict__ src,
+                            int Cs, int s0, int C, long V) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V * C) return;
+  const long v = i / C;
+  const int c = (int)(i - v * C);
+  dst[v * Cd + d0 + c] = src[v * Cs + s0 + c];
+}
+
+// residual 1x1x1 head: out[v][k] = inp[v][k] + b[k] + sum_c w[k][c] x[v][c]
+__global__ void k_head(float* __restrict__ out, const bf16* __restrict__ x,
+                       const float* __restrict__ w, const float* __restrict__ b,
+                       const float* __restrict__ inp, long V, int C, int K) {
+  const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  for (int k = 0; k < K; ++k) {
+    float s = b[k] + (inp ? inp[v * K + k] : 0.f);
+    for (int c = 0; c < C; ++c) s += w[k * C + c] * bf2f(x[v * C + c]);
+    out[v * K + k] = s;
+  }
+}
+// head backward: dx[v][c] = sum_k dout[v][k] w[k][c] (bf16); per-block
+// partials of dw[k][c] = sum_v dout[v][k] x[v][c] and db[k] = sum_v dout[v][k],
+// interleaved as part[b][k*(C+1) + c], c = C for the bias
+template <int C, int K>
+__global__ __launch_bounds__(256) void k_head_bwd(bf16* __restrict__ dx, double* __restrict__ part,
+                                                  const float* __restrict__ dout,
+                                                  const bf16* __restrict__ x,
+                                                  const float* __restrict__ w, long V) {
+  constexpr int NA = K * (C + 1);
+  double acc[NA];
+#pragma unroll
+  for (int j = 0; j < NA; ++j) acc[j] = 0.0;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (long)gridDim.x * blockDim.x) {
+    float g[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) g[k] = dout[v * K + k];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      float s = 0.f;
+#pragma unroll
+      for (int k = 0; k < K; ++k) s += g[k] * w[k * C + c];
+      dx[v * C + c] = f2bf(s);
+      const double xv = bf2f(x[v * C + c]);
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc[k * (C + 1) + c] += (double)g[k] * xv;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k * (C + 1) + C] += g[k];
+  }
+  __shared__ double sm[4];
+#pragma unroll
+  for (int j = 0; j < NA; ++j) {
+    const double s = block_sum1(acc[j], sm);
+    if (threadIdx.x == 0) part[(long)blockIdx.x * NA + j] = s;
+  }
+}
+__global__ void k_sum_blocks(float* __restrict__ out, const double* __restrict__ part, int nblk,
+                             int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[(long)b * n + j];
+  out[j] = (float)s;
+}
+
+// mean squared error: per-block FP64 partial of sum r^2, grad = 2 r / n
+__global__ __launch_bounds__(256) void k_mse(double* __restrict__ part, float* __restrict__ grad,
+                                             const float* __restrict__ pred,
+                                             const float* __restrict__ target, long n) {
+  double s = 0.0;
+  const float sc = (float)(2.0 / (double)n);
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const float r = pred[i] - target[i];
+    s += (double)r * r;
+    if (grad) grad[i] = sc * r;
+  }
+  __shared__ double sm[4];
+  s = block_sum1(s, sm);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Adam on FP32 master weights; refreshes the bf16 copy used by the
+// convolution ([Cout][Kp], k < K) or, for K == 0, an FP32 copy is not kept.
+__global__ void k_adam(float* __restrict__ wm, bf16* __restrict__ wb, const float* __restrict__ g,
+                       float* __restrict__ m, float* __restrict__ v, long n, int K, int Kp,
+                       float lr, float b1, float b2, float eps, float c1, float c2) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wn = wm[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  wm[i] = wn;
+  if (wb) {
+    const long co = i / K, k = i - co * K;
+    wb[co * Kp + k] = f2bf(wn);
+  }
+}
+
+// FP32 master weights [Cout][K] -> bf16 [Cout][Kp] (zero padding)
+__global__ void k_pack_w(bf16* __restrict__ wb, const float* __restrict__ wm, int Cout, int K,
+                         int Kp) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)Cout * Kp) return;
+  const int co = (int)(i / Kp), k = (int)(i - (long)co * Kp);
+  wb[i] = k < K ? f2bf(wm[(long)co * K + k]) : (bf16)0;
+}
+
+// float32 -> bf16 and back (network input, gradients)
+__global__ void k_to_bf16(bf16* __restrict__ y, const float* __restrict__ x, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+__global__ void k_to_f32(float* __restrict__ y, const bf16* __restrict__ x, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = bf2f(x[i]);
+}
+
+// one MFMA on a 16x16x16 tile (operand-layout check): c = a (16x16) * b (16x16),
+// a, b row-major bf16, c row-major float
+__global__ void k_mfma_tile(float* __restrict__ c, const bf16* __restrict__ a,
+                            const bf16* __restrict__ b) {
+  const int l = threadIdx.x, col = l & 15, kq = l >> 4;
+  short ea[4], eb[4];
+  for (int e = 0; e < 4; ++e) {
+    ea[e] = (short)a[col * 16 + 4 * kq + e];   // A[i = col][k]
+    eb[e] = (short)b[(4 * kq + e) * 16 + col];  // B[k][j = col]
+  }
+  f4v acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(s4v{ea[0], ea[1], ea[2], ea[3]},
+                                                 s4v{eb[0], eb[1], eb[2], eb[3]}, acc, 0, 0, 0);
+  for (int e = 0; e < 4; ++e) c[(4 * kq + e) * 16 + col] = acc[e];
+}
+
+inline unsigned nblocks(long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+template <int MT, int NT>
+int launch_conv(bf16* y, const bf16* x, const bf16* w, const float* b, int D, int H, int W,
+                int Cin, int Kp, int relu, hipStream_t st) {
+  const long V = (long)D * H * W;
+  const long waves = (V + 16 * NT - 1) / (16 * NT);
+  const unsigned grid = (unsigned)((waves + 3) / 4);
+  if (Cin % 16 == 0)
+    hipLaunchKernelGGL((k_conv3d<MT, NT, true>), dim3(grid), dim3(256), 0, st, y, x, w, b, D, H, W,
+                       Cin, Kp, relu);
+  else
+    hipLaunchKernelGGL((k_conv3d<MT, NT, false>), dim3(grid), dim3(256), 0, st, y, x, w, b, D, H,
+                       W, Cin, Kp, relu);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+}  // namespace
+}  // namespace pt
+
+using namespace pt;
+
+extern "C" int pt_nn_conv3d(void* y, const void* x, const void* w, const float* bias, int D, int H,
+                            int W, int Cin, int Cout, int relu, pt_stream stream) {
+  PT_REQUIRE(y && x && w, PT_ERR_ARG, "pt_nn_conv3d: null argument");
+  PT_REQUIRE(D > 0 && H > 0 && W > 0 && Cin > 0, PT_ERR_SIZE, "pt_nn_conv3d: bad sizes");
+  const int Kp = (27 * Cin + 15) / 16 * 16;
+  bf16* yy = (bf16*)y;
+  const bf16* xx = (const bf16*)x;
+  const bf16* ww = (const bf16*)w;
+  hipStream_t st = (hipStream_t)stream;
+  switch (Cout) {
+    case 16: return launch_conv<1, 4>(yy, xx, ww, bias, D, H, W, Cin, Kp, relu, st);
+    case 32: return launch_conv<2, 4>(yy, xx, ww, bias, D, H, W, Cin, Kp, relu, st);
+    case 64: return launch_conv<4, 2>(yy, xx, ww, bias, D, H, W, Cin, Kp, relu, st);
+    default: set_error("pt_nn_conv3d: Cout must be 16, 32 or 64"); return PT_ERR_SIZE;
+  }
+}
+
+extern "C" int pt_nn_conv3d_wflip(void* wt, const void* w, int Cin, int Cout, pt_stream stream) {
+  PT_REQUIRE(wt && w, PT_ERR_ARG, "pt_nn_conv3d_wflip: null argument");
+  const int Kp = (27 * Cin + 15) / 16 * 16, Kpt = (27 * Cout + 15) / 16 * 16;
+  hipLaunchKernelGGL(k_wflip, dim3(nblocks((long)Cin * Kpt)), dim3(256), 0, (hipStream_t)stream,
+                     (bf16*)wt, (const bf16*)w, Cin, Cout, Kp, Kpt);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_nn_conv3d_wgrad(float* dw, float* dbias, float* part, long part_cap,
+                                  double* bpart, const void* dy, const void* x, int D, int H,
+                                  int W, int Cin, int Cout, pt_stream stream) {
+  PT_REQUIRE(dw && part && dy && x, PT_ERR_ARG, "pt_nn_conv3d_wgrad: null argument");
+  PT_REQUIRE(Cout % 16 == 0 && Cout <= 64, PT_ERR_SIZE, "pt_nn_conv3d_wgrad: Cout 16/32/64");
+  hipStream_t st = (hipStream_t)stream;
+  const long V = (long)D * H * W;
+  const int K = 27 * Cin, Np = (K + 15) / 16 * 16;
+  const int tiles = (Cout / 16) * (Np / 16);
+  // splits: enough wavefronts to fill the device, partials within part_cap
+  long nsplit = std::max<long>(1, std::min<long>(256, 16384 / std::max(tiles, 1)));
+  nsplit = std::min<long>(nsplit, std::max<long>(1, part_cap / ((long)Cout * Np)));
+  nsplit = std::min<long>(nsplit, (V + 15) / 16);
+  long chunk = (V + nsplit - 1) / nsplit;
+  chunk = (chunk + 15) / 16 * 16;
+  nsplit = (V + chunk - 1) / chunk;
+  hipLaunchKernelGGL(k_conv3d_wgrad, dim3((tiles + 3) / 4, (unsigned)nsplit), dim3(256), 0, st,
+                     part, (const bf16*)dy, (const bf16*)x, D, H, W, Cin, Cout, Np, chunk);
+  PT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_wgrad_finish, dim3(nblocks((long)Cout * K)), dim3(256), 0, st, dw, part,
+                     (int)nsplit, Cout, Np, K);
+  PT_LAUNCH_CHECK();
+  if (dbias) {
+    PT_REQUIRE(bpart, PT_ERR_ARG, "pt_nn_conv3d_wgrad: bias partials needed");
+    const int nb = 256;
+    hipLaunchKernelGGL(k_chan_sum<bf16>, dim3(nb), dim3(256), 0, st, bpart, (const bf16*)dy, V,
+                       Cout);
+    PT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_chan_finish, dim3(1), dim3(64), 0, st, dbias, bpart, nb, Cout);
+    PT_LAUNCH_CHECK();
+  }
+  return PT_OK;
+}
+
+extern "C" int pt_nn_relu_bwd(void* dpre, const void* dy, const void* y, long n, pt_stream stream) {
+  PT_REQUIRE(dpre && dy && y, PT_ERR_ARG, "pt_nn_relu_bwd: null argument");
+  hipLaunchKernelGGL(k_relu_bwd, dim3(nblocks(n)), dim3(256), 0, (hipStream_t)stream, (bf16*)dpre,
+                     (const bf16*)dy, (const bf16*)y, n);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_nn_pool(void* y, const void* x, int D, int H, int W, int C, pt_stream stream) {
+  PT_REQUIRE(y && x && D % 2 == 0 && H % 2 == 0 && W % 2 == 0, PT_ERR_ARG, "pt_nn_pool: bad args");
+  const long n = (long)(D / 2) * (H / 2) * (W / 2) * C;
+  hipLaunchKernelGGL(k_pool, dim3(nblocks(n)), dim3(256), 0, (hipStream_t)stream, (bf16*)y,
+                     (const bf16*)x, D, H, W, C);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_nn_pool_bwd(void* dx, const void* dy, const void* add, int D, int H, int W,
+                              int C, pt_stream stream) {
+  PT_REQUIRE(dx && dy && D % 2 == 0 && H % 2 == 0 && W % 2 == 0, PT_ERR_ARG,
+             "pt_nn_pool_bwd: bad args");
+  const long n = (long)D * H * W * C;
+  hipLaunchKernelGGL(k_pool_bwd, dim3(nblocks(n)), dim3(256), 0, (hipStream_t)stream, (bf16*)dx,
+                     (const bf16*)dy, (const bf16*)add, D, H, W, C);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_nn_up(void* y, const void* x, int d, int h, int w, int C, int Ct, int c0,
+                        pt_stream stream) {
+  PT_REQUIRE(y && x && c0 >= 0 && c0 + C <= Ct, PT_ERR_ARG, "pt_nn_up: bad args");
+  const long n = 8L * d * h * w * C;
+  hipLaunchKernelGGL(k_up, dim3(nblocks(n)), dim3(256), 0, (hipStream_t)stream, (bf16*)y,
+                     (const bf16*)x, d, h, w, C, Ct, c0);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_nn_up_bwd(void* dx, const void* dy, int d, int h, int w, int C, int Ct, int c0,
+                            pt_stream stream) {
+  PT_REQUIRE(dx && dy && c0 >= 0 && c0 + C <= Ct, PT_ERR_ARG, "pt_nn_up_bwd: bad args");
+  const long n = (long)d * h * w * C;
+  hipLaunchKernelGGL(k_up_bwd, dim3(nblocks(n)), dim3(256), 0, (hipStream_t)stream, (bf16*)dx,
+                     (const bf16*)dy, d, h, w, C, Ct, c0);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_nn_chan_copy(void* dst, int Cd, int d0, const void* src, int Cs, int s0, int C,
+                               long V, pt_stream stream) {
+  PT_REQUIRE(dst && src && d0 + C <= Cd && s0 + C <= Cs, PT_ERR_ARG, "pt_nn_chan_copy: bad args");
+  hipLaunchKernelGGL(k_chan_copy, dim3(nblocks(V * C)), dim3(256), 0, (hipStream_t)stream,
+                     (bf16*)dst, Cd, d0, (const bf16*)src, Cs, s0, C, V);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_nn_head(float* out, const void* x, const float* w, const float* b,
+                          const float* inp, long V, int C, int K, pt_stream stream) {
+  PT_REQUIRE(out && x && w && b, PT_ERR_ARG, "pt_nn_head: null argument");
+  hipLaunchKernelGGL(k_head, dim3(nblocks(V)), dim3(256), 0, (hipStream_t)stream, out,
+                     (const bf16*)x, w, b, inp, V, C, K);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_nn_head_bwd(void* dx, float* dwb, double* part, long part_cap,
+                              const float* dout, const void* x, const float* w, long V, int C,
+                              int K, pt_stream stream) {
+  PT_REQUIRE(dx && dwb && part && dout && x && w, PT_ERR_ARG, "pt_nn_head_bwd: null argument");
+  PT_REQUIRE(C == 16 && K == 2, PT_ERR_SIZE, "pt_nn_head_bwd: built for C = 16, K = 2");
+  constexpr int NA = 2 * 17;
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = (int)std::min<long>(256, std::max<long>(1, part_cap / NA));
+  hipLaunchKernelGGL((k_head_bwd<16, 2>), dim3(nb), dim3(256), 0, st, (bf16*)dx, part, dout,
+                     (const bf16*)x, w, V);
+  PT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum_blocks, dim3(1), dim3(64), 0, st, dwb, part, nb, NA);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_nn_mse(double* part, int nblk, float* grad, const float* pred,
+                         const float* target, long n, pt_stream stream) {
+  PT_REQUIRE(part && pred && target && nblk > 0, PT_ERR_ARG, "pt_nn_mse: bad args");
+  hipLaunchKernelGGL(k_mse, dim3(nblk), dim3(256), 0, (hipStream_t)stream, part, grad, pred, target,
+                     n);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_nn_adam(float* wm, void* wb, const float* g, float* m, float* v, long n, int K,
+                          int Kp, float lr, float b1, float b2, float eps, int t,
+                          pt_stream stream) {
+  PT_REQUIRE(wm && g && m && v && t >= 1, PT_ERR_ARG, "pt_nn_adam: bad args");
+  PT_REQUIRE(!wb || (K > 0 && Kp >= K), PT_ERR_ARG, "pt_nn_adam: bad packing");
+  const float c1 = (float)(1.0 - std::pow((double)b1, t)), c2 = (float)(1.0 - std::pow((double)b2, t));
+  hipLaunchKernelGGL(k_adam, dim3(nblocks(n)), dim3(256), 0, (hipStream_t)stream, wm, (bf16*)wb, g,
+                     m, v, n, K, Kp, lr, b1, b2, eps, c1, c2);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_nn_pack_w(void* wb, const float* wm, int Cout, int Cin, pt_stream stream) {
+  PT_REQUIRE(wb && wm, PT_ERR_ARG, "pt_nn_pack_w: null argument");
+  const int K = 27 * Cin, Kp = (K + 15) / 16 * 16;
+  hipLaunchKernelGGL(k_pack_w, dim3(nblocks((long)Cout * Kp)), dim3(256), 0, (hipStream_t)stream,
+                     (bf16*)wb, wm, Cout, K, Kp);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_nn_to_bf16(void* y, const float* x, long n, pt_stream stream) {
+  PT_REQUIRE(y && x, PT_ERR_ARG, "pt_nn_to_bf16: null argument");
+  hipLaunchKernelGGL(k_to_bf16, dim3(nblocks(n)), dim3(256), 0, (hipStream_t)stream, (bf16*)y, x, n);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_nn_to_f32(float* y, const void* x, long n, pt_stream stream) {
+  PT_REQUIRE(y && x, PT_ERR_ARG, "pt_nn_to_f32: null argument");
+  hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(256), 0, (hipStream_t)stream, y,
+                     (const bf16*)x, n);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_nn_mfma_tile(float* c, const void* a, const void* b, pt_stream stream) {
+  PT_REQUIRE(c && a && b, PT_ERR_ARG, "pt_nn_mfma_tile: null argument");
+  hipLaunchKernelGGL(k_mfma_tile, dim3(1), dim3(64), 0, (hipStream_t)stream, c, (const bf16*)a,
+                     (const bf16*)b);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}

@@ -1,0 +1,169 @@
+"""Learned warm start of ADMM (no counterpart in the reference).
+
+The U-Net (``unet.UNet3D``) maps the (delta, beta) volume after a few ADMM
+iterations to the volume after many; its output replaces u in the short
+run's state (h and e are recomputed from it: h = exptomo(R u) e^{i pshift},
+e = J u) and ADMM continues from there.  Plain ADMM stays the fallback and
+the judge of the error: ``compare`` reports the time to reach the plain
+method's converged Lagrangian within a relative tolerance, for both.
+
+Training pairs are made by the solver itself on seeded ellipsoid phantoms
+(``pairs``): input = u after ``n_short`` iterations, target = u after
+``n_long`` iterations of the same run.  Volumes are normalised per sample by
+the input's largest magnitude.
+"""
+import time
+
+import numpy as np
+import torch
+
+from ._dev import c64, ptr, stream
+from . import _lib
+from .synthetic import Workload, initial_state, make_workload
+from .unet import UNet3D
+
+
+def workload(n=128, seed=0, photons=None, nell=None):
+    """Seeded n^3 phantom workload with 1.5 n angles, 16x16 probe, step 4;
+    peak photon count log-uniform in [1e1, 1e4] from the seed unless given."""
+    rng = np.random.default_rng(10_000 + seed)
+    ph = float(photons) if photons else float(10 ** rng.uniform(1, 4))
+    nth = (3 * n // 2 + 7) // 8 * 8
+    return Workload(f"warm-start {n}^3", n, n, nth, nell or max(8, n // 3), seed=seed,
+                    photons=ph, ptheta=nth // 8)
+
+
+def run_kwargs(w):
+    return dict(alpha=w.alpha, piter=w.piter, titer=w.titer, model=w.model, verbose=False)
+
+
+def to_net(u):
+    """complex [nz, n, n] -> (float32 [nz, n, n, 2] normalised, scale)."""
+    x = torch.stack([u.real, u.imag], -1).to(torch.float32).contiguous()
+    s = float(x.abs().max().item())
+    s = 1.0 / s if s > 0 else 1.0
+    return (x * s).contiguous(), s
+
+
+def from_net(y, s):
+    return torch.complex(y[..., 0], y[..., 1]).to(c64).contiguous() / s
+
+
+def pairs(seeds, n=128, n_short=4, n_long=100, log=None):
+    """Training pairs (input, target) as normalised float32 volumes."""
+    out = []
+    for sd in seeds:
+        w = workload(n, sd)
+        t0 = time.perf_counter()
+        slv, data, _ = make_workload(w)
+        st = initial_state(w)
+        kw = run_kwargs(w)
+        slv.admm(data, st["h"], st["e"], st["psi"], st["phi"], st["lamd"], st["mu"], st["u"],
+                 niter=n_short, lagr_every=0, **kw)
+        u_s = slv.state["u"].clone()
+        slv.admm_resume(data, niter=n_long - n_short, lagr_every=0, **kw)
+        u_l = slv.state["u"].clone()
+        x, s = to_net(u_s)
+        y = torch.stack([u_l.real, u_l.imag], -1).to(torch.float32) * s
+        out.append((x, y.contiguous()))
+        if log:
+            log(f"pair seed {sd} photons {w.photons:.3g}: {time.perf_counter() - t0:.1f}s")
+        del slv, data
+    return out
+
+
+def train(net, data, epochs=20, lr=1e-3, log=None):
+    """Adam on the pairs, lr decayed by 10 over the run; returns the losses."""
+    losses = []
+    for ep in range(epochs):
+        lr_e = lr * (0.1 ** (ep / max(epochs - 1, 1)))
+        tot = 0.0
+        for x, y in data:
+            tot += float(net.train_step(x, y, lr=lr_e).item())
+        losses.append(tot / len(data))
+        if log:
+            log(f"epoch {ep} loss {losses[-1]:.4e}")
+    return losses
+
+
+def warm_state(slv, state, u):
+    """Short-run state with u replaced by the network's estimate: R u, h and
+    e recomputed (ADMM's own h and e updates, solver.py:321-324)."""
+    s = dict(state)
+    s["u"] = u.contiguous()
+    s["Ru"] = slv.fwd_tomo_batch(s["u"])
+    h = torch.empty_like(s["psi"])
+    c = slv.voxelsize * slv.wavenumber() / float(slv.coeftomo)
+    pshift = slv.history[-1]["pshift"] if slv.history else 0.0
+    _lib.call("pt_k_exptomo", ptr(h), ptr(s["Ru"]), c, float(pshift), None, h.numel(), stream())
+    s["h"] = h
+    s["e"] = slv.fwd_reg(s["u"])
+    return s
+
+
+def _lagr_trace(slv, data, w, niter, state=None, m0=0):
+    """ADMM with the Lagrangian every iteration; returns (times, lagr[6])."""
+    kw = run_kwargs(w)
+    stamps, lag = [], []
+
+    def cb(m, s):
+        torch.cuda.synchronize()
+        stamps.append(time.perf_counter())
+
+    if state is None:
+        st = initial_state(w)
+        slv.admm(data, st["h"], st["e"], st["psi"], st["phi"], st["lamd"], st["mu"], st["u"],
+                 niter=niter, lagr_every=1, callback=cb, **kw)
+    else:
+        slv.state = dict(state, m=m0)
+        slv.admm_resume(data, niter=niter, lagr_every=1, callback=cb, **kw)
+    lag = [r["lagr"][6] for r in slv.history]
+    return np.array(stamps), np.array(lag)
+
+
+def time_to(stamps, lag, final, tol, t0):
+    """Wall time from t0 until lagr stays within tol of final (inf if never)."""
+    ok = np.abs(lag - final) <= tol * abs(final)
+    if not ok[-1]:
+        return float("inf"), None
+    k = len(ok) - 1
+    while k > 0 and ok[k - 1]:
+        k -= 1
+    return float(stamps[k] - t0), int(k)
+
+
+def compare(net, w, n_ref=200, n_short=4, n_max=200, tol=1e-3, log=None):
+    """Plain ADMM vs warm start on workload w.  The plain run's Lagrangian
+    after n_ref iterations is the converged value; both runs are timed
+    until their Lagrangian stays within tol of it.  The warm start's time
+    includes its n_short plain iterations and the network."""
+    slv, data, _ = make_workload(w)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stamps, lag = _lagr_trace(slv, data, w, n_ref)
+    final = float(lag[-1])
+    t_plain, k_plain = time_to(stamps, lag, final, tol, t0)
+    # warm start: n_short plain iterations, the network, then ADMM
+    kw = run_kwargs(w)
+    st = initial_state(w)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    slv.admm(data, st["h"], st["e"], st["psi"], st["phi"], st["lamd"], st["mu"], st["u"],
+             niter=n_short, lagr_every=0, **kw)
+    x, s = to_net(slv.state["u"])
+    u_w = from_net(net.forward(x), s)
+    ws = warm_state(slv, slv.state, u_w)
+    stamps_w, lag_w = _lagr_trace(slv, data, w, n_max, state=ws, m0=n_short)
+    t_warm, k_warm = time_to(stamps_w, lag_w, final, tol, t1)
+    res = {"final_lagr": final, "tol": tol, "plain_s": t_plain, "plain_iters": k_plain,
+           "warm_s": t_warm, "warm_iters": None if k_warm is None else k_warm + n_short,
+           "speedup": (t_plain / t_warm) if np.isfinite(t_warm) and t_warm > 0 else 0.0,
+           "fallback_to_plain": not np.isfinite(t_warm),
+           "lagr_plain_tail": [float(v) for v in lag[-3:]],
+           "lagr_warm_tail": [float(v) for v in lag_w[-3:]]}
+    if log:
+        log(f"compare: {res}")
+    return res
+
+
+__all__ = ["workload", "pairs", "train", "warm_state", "compare", "UNet3D"]

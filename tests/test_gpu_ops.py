@@ -178,7 +178,7 @@ def test_solver_fbp_inverts_radon():
     """Solver.fbp_tomo_batch(fwd_tomo_batch(u)) approximates u (inverse
     Radon transform; error of the reference's ramp filter itself ~9 %)."""
     from ptychotomo_amd import Solver, objects
-    n, nz, ntheta = 64, 4, 96
+    n, nz, ntheta = 64, 16, 96
     theta = np.linspace(0, np.pi, ntheta, endpoint=False).astype(np.float32)
     prb = objects.gaussian_probe(16, 4.0, 0.5)
     scan = np.zeros((2, ntheta, 4), np.float32)

@@ -1,0 +1,224 @@
+"""GPU checks of the warm-start U-Net kernels against the FP64 oracle
+(oracle/unet.py) on bf16-rounded inputs: the matrix-core operand layout, the
+convolution and its input/weight/bias gradients, the whole network forward
+and backward, and that training lowers the loss.
+
+Tolerances: a convolution output is rounded to bf16 (relative 2^-9), so
+single layers are compared at 1e-2 of the output's max; FP32-accumulated
+weight gradients at 1e-5; the whole network at 2e-2 relative L2 against the
+oracle evaluated with the GPU's bf16 rounding points, and at 1.5e-1 against
+the plain FP64 oracle (about ten bf16 roundings deep)."""
+import numpy as np
+import pytest
+
+from oracle import unet as on
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def bf16_bits(a):
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16).view(np.int16)
+
+
+def bf16_round(a):
+    b = bf16_bits(a).view(np.uint16).astype(np.uint32) << 16
+    return b.view(np.float32).astype(np.float64)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def relerr(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def from_bf16(t):
+    u = t.cpu().numpy().view(np.uint16).astype(np.uint32) << 16
+    return u.view(np.float32).astype(np.float64)
+
+
+def test_mfma_operand_layout():
+    from ptychotomo_amd import _lib
+    from ptychotomo_amd._dev import ptr, stream
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((16, 16))
+    b = rng.standard_normal((16, 16))
+    c = torch.zeros((16, 16), dtype=torch.float32, device="cuda")
+    ad, bd = dev(bf16_bits(a)), dev(bf16_bits(b))
+    _lib.call("pt_nn_mfma_tile", ptr(c), ptr(ad), ptr(bd), stream())
+    ref = bf16_round(a) @ bf16_round(b)
+    assert relerr(c.cpu().numpy(), ref) < 1e-6
+
+
+def _pack(w):
+    """[Cout, 27, Cin] -> bf16 bits [Cout][Kp]"""
+    co, _, ci = w.shape
+    kp = (27 * ci + 15) // 16 * 16
+    out = np.zeros((co, kp), np.float32)
+    out[:, :27 * ci] = w.reshape(co, 27 * ci)
+    return bf16_bits(out)
+
+
+@pytest.mark.parametrize("dims,cin,cout", [((4, 6, 16), 2, 16), ((4, 4, 8), 16, 32),
+                                           ((2, 6, 5), 48, 16), ((4, 4, 4), 32, 64),
+                                           ((2, 4, 32), 96, 32)])
+def test_conv_forward_and_gradients(dims, cin, cout):
+    from ptychotomo_amd import _lib
+    from ptychotomo_amd._dev import ptr, stream
+    print(f"conv dims={dims} cin={cin} cout={cout}", flush=True)
+    rng = np.random.default_rng(cin + cout)
+    D, H, W = dims
+    V = D * H * W
+    x = bf16_round(rng.standard_normal((D, H, W, cin)))
+    w = bf16_round(rng.standard_normal((cout, 27, cin)) / np.sqrt(27 * cin))
+    b = rng.standard_normal(cout).astype(np.float32)
+    xd, wd, bd = dev(bf16_bits(x)), dev(_pack(w)), dev(b)
+    st = stream()
+    y = torch.zeros(V * cout, dtype=torch.int16, device="cuda")
+    for relu in (0, 1):
+        _lib.call("pt_nn_conv3d", ptr(y), ptr(xd), ptr(wd), ptr(bd), D, H, W, cin, cout, relu, st)
+        ref = on.conv3d(x, w, b.astype(np.float64))
+        if relu:
+            ref = on.relu(ref)
+        got = from_bf16(y).reshape(D, H, W, cout)
+        assert np.max(np.abs(got - ref)) <= 1e-2 * np.max(np.abs(ref)), relu
+    # weight and bias gradients (FP32 accumulation of bf16 products)
+    dy = bf16_round(rng.standard_normal((D, H, W, cout)))
+    dyd = dev(bf16_bits(dy))
+    dw = torch.zeros(cout * 27 * cin, dtype=torch.float32, device="cuda")
+    db = torch.zeros(cout, dtype=torch.float32, device="cuda")
+    part = torch.zeros(1 << 20, dtype=torch.float32, device="cuda")
+    bpart = torch.zeros(256 * 64, dtype=torch.float64, device="cuda")
+    _lib.call("pt_nn_conv3d_wgrad", ptr(dw), ptr(db), ptr(part), part.numel(), ptr(bpart),
+              ptr(dyd), ptr(xd), D, H, W, cin, cout, st)
+    rdw, rdb = on.conv3d_dw(dy, x)
+    assert relerr(dw.cpu().numpy().reshape(cout, 27, cin), rdw) < 1e-5
+    assert relerr(db.cpu().numpy(), rdb) < 1e-5
+    # input gradient through the flipped weights (Cout of the flipped conv: cin)
+    if cin % 16 == 0 and cin in (16, 32, 64):
+        kpt = (27 * cout + 15) // 16 * 16
+        wt = torch.zeros(cin * kpt, dtype=torch.int16, device="cuda")
+        _lib.call("pt_nn_conv3d_wflip", ptr(wt), ptr(wd), cin, cout, st)
+        dx = torch.zeros(V * cin, dtype=torch.int16, device="cuda")
+        _lib.call("pt_nn_conv3d", ptr(dx), ptr(dyd), ptr(wt), None, D, H, W, cout, cin, 0, st)
+        ref = on.conv3d_dx(dy, w)
+        got = from_bf16(dx).reshape(D, H, W, cin)
+        assert np.max(np.abs(got - ref)) <= 1e-2 * np.max(np.abs(ref))
+
+
+def _oracle_params(net):
+    p = {}
+    for name in on.CONVS:
+        p[name] = (bf16_round(net.w[name].cpu().numpy()), net.b[name].cpu().numpy().astype(np.float64))
+    p["head"] = (net.w["head"].cpu().numpy().astype(np.float64),
+                 net.b["head"].cpu().numpy().astype(np.float64))
+    return p
+
+
+def _fwd_bf16(p, x):
+    """oracle forward with the GPU's bf16 rounding points (conv and pooling
+    outputs, the network input); the head is FP32 with the FP32 input."""
+    a = {}
+    R = bf16_round
+
+    def cr(name, inp):
+        w, b = p[name]
+        a[name + "_in"] = inp
+        y = R(on.relu(on.conv3d(inp, w, b)))
+        a[name] = y
+        return y
+    xb = R(x)
+    e1 = cr("e1b", cr("e1a", xb))
+    e2 = cr("e2b", cr("e2a", R(on.avgpool2(e1))))
+    bn = cr("bnb", cr("bna", R(on.avgpool2(e2))))
+    d2 = cr("d2b", cr("d2a", np.concatenate([on.upsample2(bn), e2], -1)))
+    d1 = cr("d1b", cr("d1a", np.concatenate([on.upsample2(d2), e1], -1)))
+    a["d1"] = d1
+    return on.head(d1, *p["head"], x), a
+
+
+def _bwd_bf16(p, a, dout, c=(16, 32, 64)):
+    """oracle backward with the GPU's bf16 rounding points (input gradients
+    of convolutions, pooling, upsampling and the head)."""
+    R = bf16_round
+    g = {}
+    dd1, hw, hb = on.head_dx(dout, p["head"][0], a["d1"])
+    g["head"] = (hw, hb)
+
+    def cb(name, dy, need=True):
+        dpre = on.relu_dx(dy, a[name])
+        g[name] = on.conv3d_dw(dpre, a[name + "_in"])
+        return R(on.conv3d_dx(dpre, p[name][0])) if need else None
+    c1, c2, c3 = c
+    dcat1 = cb("d1a", cb("d1b", R(dd1)))
+    dd2 = R(on.upsample2_dx(dcat1[..., :c2]))
+    dcat2 = cb("d2a", cb("d2b", dd2))
+    dbn = R(on.upsample2_dx(dcat2[..., :c3]))
+    dp2 = cb("bna", cb("bnb", dbn))
+    de2 = R(dcat2[..., c3:] + on.avgpool2_dx(dp2))
+    dp1 = cb("e2a", cb("e2b", de2))
+    de1 = R(dcat1[..., c2:] + on.avgpool2_dx(dp1))
+    cb("e1a", cb("e1b", de1), need=False)
+    return g
+
+
+def test_network_forward_backward_vs_oracle():
+    """Whole network against the oracle with the GPU's bf16 rounding points
+    (tight), and against the plain FP64 oracle (loose: ~10 bf16 roundings
+    deep, the first layer's gradient differs by ~5 %)."""
+    from ptychotomo_amd.unet import UNet3D
+    print("unet fwd/bwd", flush=True)
+    rng = np.random.default_rng(5)
+    net = UNet3D(seed=3)
+    for name in net.b:
+        net.b[name].copy_(torch.from_numpy(0.05 * rng.standard_normal(net.b[name].shape)
+                                           .astype(np.float32)))
+    net.w["head"].mul_(30.0)            # make the head's contribution visible
+    x = rng.standard_normal((8, 8, 8, 2)).astype(np.float32)
+    target = rng.standard_normal((8, 8, 8, 2)).astype(np.float32)
+    out = net.forward(dev(x))
+    loss, gr = net.loss(out, dev(target))
+    net.backward(gr)
+    p = _oracle_params(net)
+    x64 = x.astype(np.float64)
+    got = out.cpu().numpy()
+    _, dout = on.mse(got.astype(np.float64), target.astype(np.float64))
+    ro, acts = _fwd_bf16(p, x64)
+    e = relerr(got - x, ro - x64)
+    rf, af = on.forward(p, bf16_round(x))
+    ef = relerr(got - x, rf - bf16_round(x))
+    print(f"head contribution: vs bf16-point oracle {e:.3e}, vs FP64 oracle {ef:.3e}")
+    assert e < 1e-2 and ef < 5e-2
+    g = _bwd_bf16(p, acts, dout)
+    gf = on.backward(p, bf16_round(x), af, dout)
+    for name in on.CONVS + ["head"]:
+        gw = net.g[name].cpu().numpy().reshape(g[name][0].shape)
+        gb = net.gb[name].cpu().numpy()
+        ew, eb = relerr(gw, g[name][0]), relerr(gb, g[name][1])
+        fw, fb = relerr(gw, gf[name][0]), relerr(gb, gf[name][1])
+        print(f"{name}: dw {ew:.2e} db {eb:.2e} (FP64 oracle: {fw:.2e} {fb:.2e})")
+        assert ew < 2e-2 and eb < 2e-2, name
+        assert fw < 1.5e-1 and fb < 1.5e-1, name
+    lr = float(loss.item())
+    assert abs(lr - np.mean((got - target) ** 2)) <= 1e-6 * lr
+
+
+def test_training_lowers_loss():
+    from ptychotomo_amd.unet import UNet3D
+    print("unet training", flush=True)
+    rng = np.random.default_rng(6)
+    net = UNet3D(seed=1)
+    x = rng.standard_normal((16, 16, 16, 2)).astype(np.float32)
+    # a smooth target the network can fit: a blurred, scaled copy of x
+    t = x.copy()
+    for ax in range(3):
+        t = 0.5 * t + 0.25 * (np.roll(t, 1, ax) + np.roll(t, -1, ax))
+    target = (0.5 * x + t).astype(np.float32)
+    xd, td = dev(x), dev(target)
+    losses = [float(net.train_step(xd, td, lr=3e-3).item()) for _ in range(40)]
+    print("loss", losses[0], losses[-1])
+    assert np.all(np.isfinite(losses)) and losses[-1] < 0.5 * losses[0]
