@@ -39,6 +39,9 @@ _SIGS = {
     "pt_tomo_fwd_scaled": (c_int, [vp, vp, vp, c_float, vp]),
     "pt_tomo_adj_scaled": (c_int, [vp, vp, vp, c_float, vp]),
     "pt_tomo_set_filter": (c_int, [vp, c_int]),
+    "pt_tomo_get_tuning": (c_int, [vp, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                                   ctypes.POINTER(c_int)]),
+    "pt_tomo_set_tuning": (c_int, [vp, c_int, c_int, c_int]),
     "pt_ptycho_create": (c_int, [c_int] * 8 + [ctypes.POINTER(vp)]),
     "pt_ptycho_destroy": (c_int, [vp]),
     "pt_ptycho_set_scan_probe": (c_int, [vp, vp, vp, vp]),

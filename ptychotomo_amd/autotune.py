@@ -1,0 +1,94 @@
+"""Launch-shape autotuner for the USFFT passes (no counterpart in the
+reference).
+
+For a tomography handle it times the forward and adjoint operators over the
+launch shapes the LDS FFTs admit -- FFT lines per workgroup of the row pass,
+the column pass and the adjoint detector pass -- one parameter at a time
+(each belongs to its own kernel), keeps the fastest, and caches the result
+per (n, ntheta, pnz) in a JSON file.  The shapes change only the launch, not
+the arithmetic: results are bit-identical across shapes.
+"""
+import ctypes
+import json
+import os
+
+import torch
+
+from . import _lib
+
+CACHE = os.environ.get("PTYCHOTOMO_TUNE_CACHE",
+                       os.path.join(os.path.expanduser("~"), ".cache", "ptychotomo_amd",
+                                    "tune.json"))
+
+
+def get(op):
+    r, c, a = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _lib.call("pt_tomo_get_tuning", op._h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(a))
+    return r.value, c.value, a.value
+
+
+def _set(op, shape):
+    return _lib.lib.pt_tomo_set_tuning(op._h, *shape) == 0
+
+
+def _time(fn, reps=3):
+    fn()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    best = float("inf")
+    for _ in range(reps):
+        ev[0].record()
+        fn()
+        ev[1].record()
+        ev[1].synchronize()
+        best = min(best, ev[0].elapsed_time(ev[1]))
+    return best
+
+
+def tune(op, nz, n, ntheta, cache=CACHE, log=None):
+    """Tune a ``radonusfft`` handle in place; returns (shape, timings)."""
+    key = f"{n}x{ntheta}x{nz}"
+    db = {}
+    if cache and os.path.exists(cache):
+        with open(cache) as fh:
+            db = json.load(fh)
+    if key in db and _set(op, tuple(db[key]["shape"])):
+        return tuple(db[key]["shape"]), db[key]
+    dev = torch.device("cuda")
+    f = torch.randn((nz, n, n), dtype=torch.complex64, device=dev)
+    g = torch.randn((ntheta, nz, n), dtype=torch.complex64, device=dev)
+    out_g = torch.empty_like(g)
+    out_f = torch.empty_like(f)
+    fwd = lambda: op.fwd(out_g.data_ptr(), f.data_ptr())
+    adj = lambda: op.adj(out_f.data_ptr(), g.data_ptr())
+    best = list(get(op))
+    timings = {"default": list(best)}
+    # one parameter at a time: rows (row pass, forward), cols (column
+    # passes, both), adjoint lines (adjoint detector pass)
+    for idx, cands, fn in ((0, [1, 2, 3, 4, 6, 8], fwd), (1, [1, 2, 4, 8], lambda: (fwd(), adj())),
+                           (2, [1, 2, 4, 8], adj)):
+        res = {}
+        for v in cands:
+            shape = list(best)
+            shape[idx] = v
+            if not _set(op, tuple(shape)):
+                continue
+            res[v] = _time(fn)
+        if res:
+            best[idx] = min(res, key=res.get)
+            timings[f"param{idx}"] = {str(k): v for k, v in res.items()}
+        _set(op, tuple(best))
+        if log:
+            log(f"autotune {key} param {idx}: {res} -> {best[idx]}")
+    entry = {"shape": best, **timings}
+    if cache:
+        os.makedirs(os.path.dirname(cache), exist_ok=True)
+        db[key] = entry
+        with open(cache, "w") as fh:
+            json.dump(db, fh, indent=1)
+    return tuple(best), entry
+
+
+def tune_solver(slv, **kw):
+    """Tune the whole-volume tomography handle of a Solver."""
+    return tune(slv._tomo, slv.nz, slv.n, slv.ntheta, **kw)

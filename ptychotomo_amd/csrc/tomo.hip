@@ -834,6 +834,33 @@ static int tomo_adj_impl(pt_tomo* h, void* f, const void* g, float scale, pt_str
   return run_adj(h, (float2*)f, (const float2*)g, scale, (hipStream_t)stream);
 }
 
+extern "C" int pt_tomo_get_tuning(pt_tomo* h, int* rows, int* cols, int* adj_lines) {
+  PT_REQUIRE(h && rows && cols && adj_lines, PT_ERR_ARG, "pt_tomo_get_tuning: null argument");
+  *rows = h->rows_per_wg;
+  *cols = h->cols_per_wg;
+  *adj_lines = h->adj_lines;
+  return PT_OK;
+}
+
+extern "C" int pt_tomo_set_tuning(pt_tomo* h, int rows, int cols, int adj_lines) {
+  PT_REQUIRE(h, PT_ERR_ARG, "pt_tomo_set_tuning: null handle");
+  const int N = h->N;
+  const size_t cap = 64 * 1024;
+  auto ok = [&](int lines) {
+    return lines >= 1 && lds_fft_fits(h->plan, lines, 256) &&
+           (size_t)lines * N * sizeof(float2) <= cap;
+  };
+  PT_REQUIRE(ok(2 * rows), PT_ERR_SIZE, "pt_tomo_set_tuning: rows per workgroup do not fit");
+  PT_REQUIRE(ok(2 * cols) && (2 * N) % cols == 0, PT_ERR_SIZE,
+             "pt_tomo_set_tuning: columns per workgroup do not fit or do not divide 2N");
+  PT_REQUIRE(ok(adj_lines) && adj_lines <= kZQ, PT_ERR_SIZE,
+             "pt_tomo_set_tuning: adjoint lines per workgroup do not fit");
+  h->rows_per_wg = rows;
+  h->cols_per_wg = cols;
+  h->adj_lines = adj_lines;
+  return PT_OK;
+}
+
 extern "C" int pt_tomo_set_filter(pt_tomo* h, int kind) {
   PT_REQUIRE(h, PT_ERR_ARG, "pt_tomo_set_filter: null handle");
   PT_REQUIRE(kind == PT_FILTER_NONE || kind == PT_FILTER_RAMP, PT_ERR_ARG,

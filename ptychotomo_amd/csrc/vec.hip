@@ -3,6 +3,7 @@
 // update_penalty, take_lagr), each as one fused pass.  Reductions accumulate
 // in FP64 with a fixed grid and a fixed-order second pass (deterministic).
 #include <cmath>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -52,22 +53,26 @@ namespace {
 constexpr int RB = 256;          // threads per reduction block
 constexpr int RMAXBLK = 512;     // blocks per segment cap
 
-// scratch for partial sums (grown on demand, one per process)
+// scratch for partial sums, one per stream (grown on demand): reductions
+// enqueued on different streams may run concurrently
+struct Partial { double* ptr = nullptr; size_t cap = 0; };
 struct Partials {
-  double* ptr = nullptr;
-  size_t cap = 0;
   std::mutex mu;
+  std::map<hipStream_t, Partial> per_stream;
 } g_part;
 
-int partial_buf(size_t n, double** out) {
+int partial_buf(size_t n, double** out, hipStream_t st) {
   std::lock_guard<std::mutex> lk(g_part.mu);
-  if (n > g_part.cap) {
-    if (g_part.ptr) PT_HIP(hipFree(g_part.ptr));
-    g_part.ptr = nullptr;
-    PT_HIP(hipMalloc(&g_part.ptr, n * sizeof(double)));
-    g_part.cap = n;
+  Partial& p = g_part.per_stream[st];
+  if (n > p.cap) {
+    // a buffer in use by queued work on this stream must outlive it
+    if (p.ptr) PT_HIP(hipStreamSynchronize(st));
+    if (p.ptr) PT_HIP(hipFree(p.ptr));
+    p.ptr = nullptr;
+    PT_HIP(hipMalloc(&p.ptr, n * sizeof(double)));
+    p.cap = n;
   }
-  *out = g_part.ptr;
+  *out = p.ptr;
   return PT_OK;
 }
 
@@ -118,7 +123,7 @@ int seg_reduce(double* out, long len, int nseg, F f, hipStream_t st) {
   int nblk = (int)std::min<long>(RMAXBLK, std::max<long>(1, (seglen + RB * 4 - 1) / (RB * 4)));
   nblk = std::max(1, std::min(nblk, std::max(1, 2048 / nseg)));
   double* part;
-  int rc = partial_buf((size_t)nseg * nblk * NV, &part);
+  int rc = partial_buf((size_t)nseg * nblk * NV, &part, st);
   if (rc) return rc;
   hipLaunchKernelGGL((k_seg_reduce<NV, F>), dim3(nblk, nseg), dim3(RB), 0, st, part, seglen, f,
                      (const double*)nullptr);
@@ -139,7 +144,7 @@ int seg_reduce_n(double* out, long len, int nseg, int nout, F f, hipStream_t st,
   int nblk = (int)std::min<long>(RMAXBLK, std::max<long>(1, (seglen + RB * 4 - 1) / (RB * 4)));
   nblk = std::max(1, std::min(nblk, std::max(1, 2048 / nseg)));
   double* part;
-  int rc = partial_buf((size_t)nseg * nblk * NV, &part);
+  int rc = partial_buf((size_t)nseg * nblk * NV, &part, st);
   if (rc) return rc;
   hipLaunchKernelGGL((k_seg_reduce<NV, F>), dim3(nblk, nseg), dim3(RB), 0, st, part, seglen, f,
                      skip);
@@ -687,7 +692,7 @@ extern "C" int pt_k_takexi_stats(double* out, const void* psi, const void* lamd,
   const long len = (long)ntheta * nz * n;
   const int nblk = (int)std::min<long>(1024, std::max<long>(1, (len + RB * 8 - 1) / (RB * 8)));
   double* part;
-  int rc = partial_buf((size_t)nblk * 3, &part);
+  int rc = partial_buf((size_t)nblk * 3, &part, st);
   if (rc) return rc;
   FTakexiStats f{(const float2*)psi, (const float2*)lamd, SV{rho, rho_dev}, nz, n, r};
   hipLaunchKernelGGL(k_takexi_stats, dim3(nblk), dim3(RB), 0, st, part, f, len);
@@ -832,7 +837,7 @@ extern "C" int pt_k_max_angle(double* out, const void* x, long len, pt_stream st
   hipStream_t st = (hipStream_t)stream;
   const int nblk = (int)std::min<long>(1024, std::max<long>(1, (len + RB * 8 - 1) / (RB * 8)));
   double* part;
-  int rc = partial_buf((size_t)nblk, &part);
+  int rc = partial_buf((size_t)nblk, &part, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_max_angle, dim3(nblk), dim3(RB), 0, st, part, (const float2*)x, len);
   PT_LAUNCH_CHECK();

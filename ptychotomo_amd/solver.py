@@ -26,8 +26,8 @@ in rounding only):
   without host round trips and can be replayed as a HIP graph.  After a zero
   step the reference's Dai-Yuan beta is 0/0; here it is 0 (restart).
 """
+import contextlib
 import ctypes
-import warnings
 
 import numpy as np
 import torch
@@ -517,59 +517,93 @@ class Solver(object):
         live state buffers.
         """
         data = self._dev_data(data)
+        ctx = self._admm_setup(h, e, psi, phi, lamd, mu, u, rho, tau, Ru, piter, titer)
+        self.history = []
+        graph_obj = None
+        # graph capture uses a side stream; the whole run is enqueued on one
+        # dedicated stream so that the library's per-stream workspaces are
+        # sized by the eager iterations before the capture
+        side = None
+        scope = contextlib.nullcontext()
+        if graph:
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            scope = torch.cuda.stream(side)
+        with scope:
+            for m in range(m0, m0 + niter):
+                if graph and m >= 2 and m > m0:
+                    if graph_obj is None:
+                        side.synchronize()
+                        graph_obj = torch.cuda.CUDAGraph()
+                        with torch.cuda.graph(graph_obj, stream=side):
+                            self._admm_step(ctx, data, m, piter, titer, alpha, model)
+                    graph_obj.replay()
+                else:
+                    self._admm_step(ctx, data, m, piter, titer, alpha, model)
+                rec = self._admm_record(ctx, data, m, piter, titer, alpha, model, lagr_every,
+                                        verbose, niter)
+                if callback is not None:
+                    S = ctx["S"]
+                    callback(m, dict(h=S["h"], e=S["e"], psi=S["psi"], phi=S["phi"],
+                                     lamd=S["lamd"], mu=S["mu"], u=S["u"], rho=rec["rho"],
+                                     tau=rec["tau"], Ru=S["Ru"]))
+        if side is not None:
+            torch.cuda.current_stream().wait_stream(side)
+        self._admm_finish(ctx, m0 + niter)
+        return ctx["S"]["u"], ctx["S"]["psi"]
+
+    # ------------------------------------------------------------------ ADMM pieces
+    def _admm_setup(self, h, e, psi, phi, lamd, mu, u, rho, tau, Ru, piter, titer):
+        """State buffers owned by one run (copies of the inputs), the device
+        scalar block and the step history buffers."""
         S = {k: to_dev(v, c64).clone() for k, v in
              dict(h=h, e=e, psi=psi, phi=phi, lamd=lamd, mu=mu, u=u).items()}
         S["Ru"] = self.fwd_tomo_batch(S["u"]) if Ru is None else to_dev(Ru, c64).clone()
         S["hn"], S["en"] = empty(S["h"].shape), empty(S["e"].shape)
         S["xi0"], S["K"], S["xi1"] = empty(S["psi"].shape), empty(S["psi"].shape), \
             empty(S["phi"].shape)
-        blk = self._scalar_block(rho, tau)
-        pmax = piter + 32
-        H = {"pg": torch.zeros((pmax, self.nchunk), dtype=f64, device=device()),
+        H = {"pg": torch.zeros((piter + 32, self.nchunk), dtype=f64, device=device()),
              "tg": torch.zeros(max(titer, 1), dtype=f64, device=device()),
              "rt": torch.zeros(2, dtype=f64, device=device())}
-        self.history = []
-        graph_obj = None
-        for m in range(m0, m0 + niter):
-            pit = piter + (m < 2) * 32
-            args = (S, blk, H, data, pit, titer, alpha, model)
-            if graph and m >= 2 and m > m0:
-                if graph_obj is None:
-                    torch.cuda.synchronize()
-                    graph_obj = torch.cuda.CUDAGraph()
-                    with torch.cuda.graph(graph_obj):
-                        self._admm_iter(*args)
-                graph_obj.replay()
-            else:
-                self._admm_iter(*args)
-            # one synchronisation per iteration: steps, penalties, phase
-            v = torch.cat([H["rt"], blk[:_NSC], H["tg"], H["pg"][:pit].reshape(-1)]).cpu().numpy()
-            rho_used, tau_used = float(v[0]), float(v[1])
-            b = v[2:2 + _NSC]
-            rho, tau = float(b[_RHO]), float(b[_TAU])
-            tg = v[2 + _NSC:2 + _NSC + H["tg"].numel()][:titer]
-            pg = v[2 + _NSC + H["tg"].numel():].reshape(pit, self.nchunk)
-            if b[_MAXANG] > 3.14:
-                print('possible phase wrap, max computed angle', float(b[_MAXANG]))
-            rec = {"m": m, "rho_used": rho_used, "tau_used": tau_used, "pshift": float(b[_PRM]),
-                   "ptycho_gammas": [[float(x) for x in pg[:, s]] for s in range(self.nchunk)],
-                   "tomo_gammas": [float(x) for x in tg], "rho": rho, "tau": tau}
-            self._chunk_gammas, self._last_tomo_gammas = rec["ptycho_gammas"], rec["tomo_gammas"]
-            if lagr_every and np.mod(m, lagr_every) == 0:
-                lagr = self.take_lagr(S["psi"], S["phi"], data, S["h"], S["e"], S["lamd"], S["mu"],
-                                      alpha, rho, tau, model)
-                rec["lagr"] = [float(x) for x in lagr]
-                if verbose:
-                    print("%d/%d) rho=%.2e, tau=%.2e, Lagrangian terms:  %.2e %.2e %.2e %.2e %.2e %.2e, Sum: %.2e" %
-                          (m, niter, rho, tau, *lagr))
-            self.history.append(rec)
-            if callback is not None:
-                callback(m, dict(h=S["h"], e=S["e"], psi=S["psi"], phi=S["phi"], lamd=S["lamd"],
-                                 mu=S["mu"], u=S["u"], rho=rho, tau=tau, Ru=S["Ru"]))
+        return dict(S=S, blk=self._scalar_block(rho, tau), H=H)
+
+    def _admm_step(self, ctx, data, m, piter, titer, alpha, model):
+        pit = piter + (m < 2) * 32      # solver.py:315 (L6)
+        self._admm_iter(ctx["S"], ctx["blk"], ctx["H"], data, pit, titer, alpha, model)
+
+    def _admm_record(self, ctx, data, m, piter, titer, alpha, model, lagr_every, verbose,
+                     niter):
+        """Read one iteration's steps and penalties (one synchronisation), the
+        Lagrangian every lagr_every iterations; appends to self.history."""
+        S, blk, H = ctx["S"], ctx["blk"], ctx["H"]
+        pit = piter + (m < 2) * 32
+        v = torch.cat([H["rt"], blk[:_NSC], H["tg"], H["pg"][:pit].reshape(-1)]).cpu().numpy()
+        rho_used, tau_used = float(v[0]), float(v[1])
+        b = v[2:2 + _NSC]
+        rho, tau = float(b[_RHO]), float(b[_TAU])
+        tg = v[2 + _NSC:2 + _NSC + H["tg"].numel()][:titer]
+        pg = v[2 + _NSC + H["tg"].numel():].reshape(pit, self.nchunk)
+        if b[_MAXANG] > 3.14:
+            print('possible phase wrap, max computed angle', float(b[_MAXANG]))
+        rec = {"m": m, "rho_used": rho_used, "tau_used": tau_used, "pshift": float(b[_PRM]),
+               "ptycho_gammas": [[float(x) for x in pg[:, s]] for s in range(self.nchunk)],
+               "tomo_gammas": [float(x) for x in tg], "rho": rho, "tau": tau}
+        self._chunk_gammas, self._last_tomo_gammas = rec["ptycho_gammas"], rec["tomo_gammas"]
+        if lagr_every and np.mod(m, lagr_every) == 0:
+            lagr = self.take_lagr(S["psi"], S["phi"], data, S["h"], S["e"], S["lamd"], S["mu"],
+                                  alpha, rho, tau, model)
+            rec["lagr"] = [float(x) for x in lagr]
+            if verbose:
+                print("%d/%d) rho=%.2e, tau=%.2e, Lagrangian terms:  %.2e %.2e %.2e %.2e %.2e %.2e, Sum: %.2e" %
+                      (m, niter, rho, tau, *lagr))
+        self.history.append(rec)
+        return rec
+
+    def _admm_finish(self, ctx, m_next):
+        S, blk = ctx["S"], ctx["blk"]
         rho, tau = (float(x) for x in blk[_RHO:_TAU + 1].cpu().numpy())
         self.state = dict(h=S["h"], e=S["e"], psi=S["psi"], phi=S["phi"], lamd=S["lamd"],
-                          mu=S["mu"], u=S["u"], rho=rho, tau=tau, Ru=S["Ru"], m=m0 + niter)
-        return S["u"], S["psi"]
+                          mu=S["mu"], u=S["u"], rho=rho, tau=tau, Ru=S["Ru"], m=m_next)
 
     def _admm_iter(self, S, blk, H, data, pit, titer, alpha, model):
         """One ADMM iteration (solver.py:312-330) on the state S in place, all

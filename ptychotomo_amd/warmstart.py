@@ -167,3 +167,62 @@ def compare(net, w, n_ref=200, n_short=4, n_max=200, tol=1e-3, log=None):
 
 
 __all__ = ["workload", "pairs", "train", "warm_state", "compare", "UNet3D"]
+
+
+def save(net, path, meta=None):
+    d = net.state_dict()
+    if meta:
+        d["meta"] = np.array(repr(meta))
+    np.savez(path, **d)
+
+
+def load(path):
+    d = dict(np.load(path, allow_pickle=False))
+    net = UNet3D()
+    net.load_state_dict(d)
+    return net
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="train / evaluate the ADMM warm-start U-Net")
+    ap.add_argument("--n", type=int, default=128, help="training volume size n^3")
+    ap.add_argument("--pairs", type=int, default=16, help="training phantoms")
+    ap.add_argument("--short", type=int, default=4, help="ADMM iterations of the input")
+    ap.add_argument("--long", type=int, default=100, help="ADMM iterations of the target")
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--out", default="unet_warmstart.npz", help="weights file to write")
+    ap.add_argument("--weights", default=None, help="weights to load instead of training")
+    ap.add_argument("--compare-n", type=int, nargs="*", default=[128],
+                    help="sizes for the plain-vs-warm comparison (held-out seed)")
+    ap.add_argument("--ref-iters", type=int, default=200)
+    ap.add_argument("--tol", type=float, default=1e-3)
+    args = ap.parse_args(argv)
+    log = lambda m: print(f"[warmstart] {m}", file=sys.stderr, flush=True)
+    res = {"n": args.n, "pairs": args.pairs, "short": args.short, "long": args.long}
+    if args.weights:
+        net = load(args.weights)
+    else:
+        t0 = time.perf_counter()
+        data = pairs(range(args.pairs), args.n, args.short, args.long, log=log)
+        res["datagen_s"] = time.perf_counter() - t0
+        net = UNet3D(seed=0)
+        t0 = time.perf_counter()
+        res["losses"] = train(net, data, args.epochs, args.lr, log=log)
+        torch.cuda.synchronize()
+        res["train_s"] = time.perf_counter() - t0
+        save(net, args.out, meta=res)
+        log(f"weights -> {args.out}")
+    res["compare"] = {}
+    for n in args.compare_n:
+        w = workload(n, seed=10_000 + n)
+        res["compare"][n] = compare(net, w, n_ref=args.ref_iters, n_short=args.short,
+                                    n_max=args.ref_iters, tol=args.tol, log=log)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

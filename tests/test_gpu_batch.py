@@ -1,0 +1,109 @@
+"""Batched reconstructions of uneven sizes (one graph launch per iteration for
+the whole batch) against each instance run alone, the per-instance FP64
+direct-NUDFT error of the tomography operator, and the launch-shape
+autotuner (shapes change the launch only: bit-identical results)."""
+import numpy as np
+import pytest
+
+from oracle import usfft as ousfft
+from ptychotomo_amd import objects
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+SIZES = [32, 48, 64]
+
+
+def _instance(n, seed):
+    ntheta = 3 * n // 2
+    theta = objects.make_theta(ntheta)
+    prb = objects.gaussian_probe(8, 2.0)
+    scan = objects.jittered_raster(ntheta, n, n, 8, 4, 1.0, np.random.default_rng(seed + 1))
+    obj = objects.ellipsoid_phantom(n, n, 6, np.random.default_rng(seed), semi=(3, n / 5))
+    args = (prb, scan, theta, [8, 8], 1e-6, 8.8, ntheta, n, n, ntheta // 4, n)
+    return args, obj
+
+
+def _init(slv):
+    sh = (slv.ntheta, slv.nz, slv.n)
+    sh3 = (3, slv.nz, slv.n, slv.n)
+    z = lambda s: torch.zeros(s, dtype=torch.complex64, device="cuda")
+    return dict(h=torch.ones(sh, dtype=torch.complex64, device="cuda"), e=z(sh3),
+                psi=torch.ones(sh, dtype=torch.complex64, device="cuda"), phi=z(sh3),
+                lamd=z(sh), mu=z(sh3), u=z((slv.nz, slv.n, slv.n)))
+
+
+def test_batch_matches_single_runs():
+    from ptychotomo_amd.batch import BatchSolver
+    from ptychotomo_amd.solver import Solver
+    print("batch", SIZES, flush=True)
+    insts = [_instance(n, 100 + i) for i, n in enumerate(SIZES)]
+    bs = BatchSolver.from_args([a for a, _ in insts])
+    datas = []
+    for slv, (_, obj) in zip(bs.solvers, insts):
+        d = slv.fwd_ptycho_batch_dev(slv.exptomo(slv.fwd_tomo_batch(obj)))
+        datas.append(torch.poisson(d * (1e2 / float(d.max()))))
+    niter = 5
+    res = bs.admm(datas, [_init(s) for s in bs.solvers], 3e-7, 4, 4, niter, "poisson",
+                  lagr_every=1)
+    for k, ((args, _), slv, d) in enumerate(zip(insts, bs.solvers, datas)):
+        one = Solver(*args)
+        st = _init(one)
+        u, psi = one.admm(d, st["h"], st["e"], st["psi"], st["phi"], st["lamd"], st["mu"],
+                          st["u"], 3e-7, 4, 4, niter, "poisson", lagr_every=1, verbose=False)
+        assert len(slv.history) == niter
+        for ra, rb in zip(slv.history, one.history):
+            assert ra == rb, (SIZES[k], ra["m"])
+        assert torch.equal(res[k][0], u) and torch.equal(res[k][1], psi), SIZES[k]
+
+
+def test_nudft_error_per_instance():
+    """Tomography operator of each batch size against the FP64 direct NUDFT
+    (error stated per instance; the USFFT targets eps = 1e-6)."""
+    from ptychotomo_amd.radonusfft import radonusfft
+    for n in SIZES:
+        ntheta = 3 * n // 2
+        theta = objects.make_theta(ntheta)
+        rng = np.random.default_rng(n)
+        u = (rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))).astype(np.complex64)
+        op = radonusfft(ntheta, 2, n)
+        th = torch.from_numpy(theta).cuda()
+        op.setobj(th.data_ptr())
+        ud = torch.from_numpy(u).cuda()
+        out = torch.zeros((ntheta, 2, n), dtype=torch.complex64, device="cuda")
+        op.fwd(out.data_ptr(), ud.data_ptr())
+        ref = ousfft.fwd_exact(u, theta)
+        err = float(np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref))
+        print(f"n={n}: USFFT vs FP64 direct NUDFT rel err {err:.2e}")
+        assert err < 1e-5
+
+
+def test_autotune_shapes_bit_identical(tmp_path):
+    from ptychotomo_amd import autotune
+    from ptychotomo_amd.radonusfft import radonusfft
+    n, nz, ntheta = 64, 4, 96
+    theta = objects.make_theta(ntheta)
+    op = radonusfft(ntheta, nz, n)
+    th = torch.from_numpy(theta).cuda()
+    op.setobj(th.data_ptr())
+    rng = np.random.default_rng(1)
+    u = torch.from_numpy((rng.standard_normal((nz, n, n)) + 1j).astype(np.complex64)).cuda()
+    g = torch.from_numpy((rng.standard_normal((ntheta, nz, n)) + 1j).astype(np.complex64)).cuda()
+    a0 = torch.empty((ntheta, nz, n), dtype=torch.complex64, device="cuda")
+    b0 = torch.empty((nz, n, n), dtype=torch.complex64, device="cuda")
+    op.fwd(a0.data_ptr(), u.data_ptr())
+    op.adj(b0.data_ptr(), g.data_ptr())
+    default = autotune.get(op)
+    shape, entry = autotune.tune(op, nz, n, ntheta, cache=str(tmp_path / "t.json"))
+    print("autotune", default, "->", shape, entry)
+    assert autotune.get(op) == shape
+    a1, b1 = torch.empty_like(a0), torch.empty_like(b0)
+    for s in [shape, (1, 1, 1), default]:
+        assert autotune._set(op, s)
+        op.fwd(a1.data_ptr(), u.data_ptr())
+        op.adj(b1.data_ptr(), g.data_ptr())
+        assert torch.equal(a0, a1) and torch.equal(b0, b1), s
+    assert not autotune._set(op, (1, 5, 1))       # 5 does not divide 2N = 192
+    # cached: a second tune reads the file
+    shape2, _ = autotune.tune(op, nz, n, ntheta, cache=str(tmp_path / "t.json"))
+    assert shape2 == shape

@@ -5,9 +5,12 @@ and backward, and that training lowers the loss.
 
 Tolerances: a convolution output is rounded to bf16 (relative 2^-9), so
 single layers are compared at 1e-2 of the output's max; FP32-accumulated
-weight gradients at 1e-5; the whole network at 2e-2 relative L2 against the
-oracle evaluated with the GPU's bf16 rounding points, and at 1.5e-1 against
-the plain FP64 oracle (about ten bf16 roundings deep)."""
+weight gradients at 1e-5; the whole network at 2e-3 relative L2 against the
+oracle evaluated with the GPU's bf16 rounding points (measured 1e-8 .. 2e-4:
+FP32 sums on the GPU and FP64 sums in the oracle round a few elements to
+neighbouring bf16 values), and
+at 3e-1 against the plain FP64 oracle (a sanity bound: about ten bf16
+roundings deep, the 2^3 bottleneck of an 8^3 volume differs by ~15 %)."""
 import numpy as np
 import pytest
 
@@ -168,8 +171,7 @@ def _bwd_bf16(p, a, dout, c=(16, 32, 64)):
 
 def test_network_forward_backward_vs_oracle():
     """Whole network against the oracle with the GPU's bf16 rounding points
-    (tight), and against the plain FP64 oracle (loose: ~10 bf16 roundings
-    deep, the first layer's gradient differs by ~5 %)."""
+    (tight), and against the plain FP64 oracle (loose sanity bound)."""
     from ptychotomo_amd.unet import UNet3D
     print("unet fwd/bwd", flush=True)
     rng = np.random.default_rng(5)
@@ -201,8 +203,8 @@ def test_network_forward_backward_vs_oracle():
         ew, eb = relerr(gw, g[name][0]), relerr(gb, g[name][1])
         fw, fb = relerr(gw, gf[name][0]), relerr(gb, gf[name][1])
         print(f"{name}: dw {ew:.2e} db {eb:.2e} (FP64 oracle: {fw:.2e} {fb:.2e})")
-        assert ew < 2e-2 and eb < 2e-2, name
-        assert fw < 1.5e-1 and fb < 1.5e-1, name
+        assert ew < 2e-3 and eb < 2e-3, name
+        assert fw < 3e-1 and fb < 3e-1, name
     lr = float(loss.item())
     assert abs(lr - np.mean((got - target) ** 2)) <= 1e-6 * lr
 
