@@ -56,7 +56,14 @@ class BatchSolver:
                             with torch.cuda.stream(s):
                                 slv._admm_step(ctx, d, m, piter, titer, alpha, model)
                             cap.wait_stream(s)
+                # the replay runs on `cur`: order it after every instance
+                # stream's previous work, and the reads of this iteration's
+                # history (on the instance streams) after it
+                for s in self.streams:
+                    cur.wait_stream(s)
                 graph_obj.replay()
+                for s in self.streams:
+                    s.wait_stream(cur)
             else:
                 for slv, ctx, d, s in zip(self.solvers, ctxs, ds, self.streams):
                     with torch.cuda.stream(s):

@@ -149,6 +149,48 @@ __device__ __forceinline__ void load_frame_row(float2 (&v)[D], const float2* __r
   }
 }
 
+// Split form of load_frame_row for kernels that issue all of a patch group's
+// global loads before the first transpose: the probe column of the lane's
+// frame row (c * prb[ix][iy], 0 outside the probe) once per wave, the raw
+// object values per patch (0 outside), and their product.
+template <int D>
+__device__ __forceinline__ void probe_row(float2 (&pr)[D], const float2* __restrict__ prb, int r,
+                                          int P, int ox, int oy, float cs) {
+  const int ix = r - ox;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const int iy = k - oy;
+    pr[k] = (ix >= 0 && ix < P && iy >= 0 && iy < P) ? cscale(prb[(long)ix * P + iy], cs)
+                                                     : make_float2(0.f, 0.f);
+  }
+}
+template <int D>
+__device__ __forceinline__ void load_rows(float2 (&v)[D], const float2* __restrict__ f, int2 ps,
+                                          int t, int r, int nz, int n, int P, int ox, int oy,
+                                          bool active) {
+  const int ix = r - ox;
+  const bool rowin = active && ps.x >= 0 && ix >= 0 && ix < P;
+  const float2* fp = f + ((long)t * nz + ps.y - oy) * n + ps.x + ix;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const int iy = k - oy;
+    v[k] = (rowin && iy >= 0 && iy < P) ? fp[(long)k * n] : make_float2(0.f, 0.f);
+  }
+}
+// v = cs * prb[ix][iy] * v, probe values read from memory (cached)
+template <int D>
+__device__ __forceinline__ void apply_probe(float2 (&v)[D], const float2* __restrict__ prb, int r,
+                                            int P, int ox, int oy, float cs) {
+  const int ix = r - ox;
+  const float2* pp = prb + (long)ix * P - oy;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const int iy = k - oy;
+    v[k] = (ix >= 0 && ix < P && iy >= 0 && iy < P) ? cscale(cmul(pp[k], v[k]), cs)
+                                                    : make_float2(0.f, 0.f);
+  }
+}
+
 // ------------------------------------------------------------------ fwd
 template <int D>
 __global__ __launch_bounds__(WAVE) void k_fwd(float2* __restrict__ g, const float2* __restrict__ f,
@@ -226,7 +268,7 @@ __global__ __launch_bounds__(WAVE) void k_minf(double* __restrict__ partial,
                                                int oy, float cs, float cp, float cd, int model,
                                                int lagr, const double* __restrict__ skip) {
   constexpr int G = WAVE / D;
-  __shared__ float2 T[G * D * (D + 1)];
+  __shared__ float T[G * D * (D + 1)];
   const int lane = lane_id(), slot = lane / D, r = lane % D;
   const int c = blockIdx.y;
   // chunks whose line search has finished (skip[2*c+1] != 0) are not evaluated
@@ -247,21 +289,28 @@ __global__ __launch_bounds__(WAVE) void k_minf(double* __restrict__ partial,
     const int t = active ? (int)(p / S) : 0;
     float2 a[D], b[D];
     float d[D];
-    load_frame_row<D>(a, f, prb, ps, t, r, nz, n, P, ox, oy, cs, active);
-    frame_fwd<D>(a, T, slot, r);
+    // every global load of the group first (one memory round trip)
+    load_rows<D>(a, f, ps, t, r, nz, n, P, ox, oy, active);
+    if (dir) load_rows<D>(b, dir, ps, t, r, nz, n, P, ox, oy, active);
+    {
+      const float* dp = data + (active ? p : 0) * D * D + r;
+#pragma unroll
+      for (int k = 0; k < D; ++k) d[k] = active ? dp[k * D] : 0.f;
+    }
+    apply_probe<D>(a, prb, r, P, ox, oy, cs);
+    frame_fwd_w<D>(a, T, slot, r);
     if (dir) {
-      load_frame_row<D>(b, dir, prb, ps, t, r, nz, n, P, ox, oy, cs, active);
-      frame_fwd<D>(b, T, slot, r);
+      apply_probe<D>(b, prb, r, P, ox, oy, cs);
+      frame_fwd_w<D>(b, T, slot, r);
     } else {
 #pragma unroll
       for (int k = 0; k < D; ++k) b[k] = make_float2(0.f, 0.f);
     }
     if (!active) continue;
-    const float* dp = data + p * D * D + r;
     float dterm = 0.f;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-      d[k] = dp[k * D] * cd;
+      d[k] = d[k] * cd;
       a[k] = cscale(a[k], cp);
       b[k] = cscale(b[k], cp);
       if (model == PT_MODEL_GAUSSIAN) {
@@ -306,25 +355,29 @@ __global__ __launch_bounds__(WAVE) void k_minf(double* __restrict__ partial,
   }
 }
 
-__global__ void k_reduce_partials(double* __restrict__ out, const double* __restrict__ partial,
-                                  int nblk, int ngam, const double* __restrict__ skip) {
-  // one thread per (chunk, gamma): fixed-order sum over blocks
-  const int c = blockIdx.x;
-  const int j = threadIdx.x;
-  if (j >= ngam) return;
+__global__ __launch_bounds__(64) void k_reduce_partials(double* __restrict__ out,
+                                                         const double* __restrict__ partial,
+                                                         int nblk, int ngam,
+                                                         const double* __restrict__ skip) {
+  // one wave per (chunk, gamma): lane-strided sums in block order, then a
+  // fixed-shape wave tree (deterministic)
+  const int c = blockIdx.x, j = blockIdx.y, lane = threadIdx.x;
   if (skip && skip[2 * c + 1] != 0.0) return;
   double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partial[((long)c * nblk + b) * ngam + j];
-  out[c * ngam + j] = s;
+  for (int b = lane; b < nblk; b += WAVE) s += partial[((long)c * nblk + b) * ngam + j];
+  s = wave_sum(s);
+  if (lane == 0) out[c * ngam + j] = s;
 }
 
 // ------------------------------------------------------------------ tiles
 enum { TILE_ADJ = 0, TILE_GRAD = 1 };
 
-// Waves per overlap-add workgroup: the waves share one LDS tile and add into
-// it in wave order (deterministic); small frames allow more waves.
+// Waves per overlap-add workgroup.  Waves of one workgroup would share the
+// LDS tile and have to add in turn (barrier-separated, deterministic); one
+// wave per tile avoids the serialisation, and small tiles keep many tiles
+// resident per CU.
 template <int D>
-constexpr int tile_waves() { return D <= 16 ? 4 : 2; }
+constexpr int tile_waves() { return 1; }
 
 template <int D, int MODE>
 __global__ __launch_bounds__(WAVE * tile_waves<D>()) void k_tile(
@@ -352,6 +405,8 @@ __global__ __launch_bounds__(WAVE * tile_waves<D>()) void k_tile(
   __syncthreads();
   const int beg = tile_start[tile], end = tile_start[tile + 1];
   const int ix = r - ox;
+  float2 pr[D];   // c * prb column of this lane's frame row (overlap-add uses conj)
+  probe_row<D>(pr, prb, r, P, ox, oy, 1.0f);
   for (int i0 = beg; i0 < end; i0 += W * G) {
     const int ii = i0 + wave * G + slot;
     const bool active = ii < end;
@@ -370,16 +425,24 @@ __global__ __launch_bounds__(WAVE * tile_waves<D>()) void k_tile(
         for (int k = 0; k < D; ++k) v[k] = make_float2(0.f, 0.f);
       }
     } else {
-      load_frame_row<D>(v, f, prb, ps, t, r, nz, n, P, ox, oy, cs, active);
+      // every global load of the group first (one memory round trip)
+      load_rows<D>(v, f, ps, t, r, nz, n, P, ox, oy, active);
+      float dv[D];
+      {
+        const float* dp = data + p * D * D + r;
+#pragma unroll
+        for (int k = 0; k < D; ++k) dv[k] = active ? dp[k * D] : 0.f;
+      }
+      const float2 sh = active ? shift[p] : make_float2(1.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < D; ++k) v[k] = cscale(cmul(pr[k], v[k]), cs);
       frame_fwd_w<D>(v, T, slot, r);
       // residual in the unshifted frame (the per-patch unimodular factor of
       // kernels.cuh:319-355 cancels: r(s*a) = s*r(a), F* removes s again)
       if (active) {
-        const float* dp = data + p * D * D + r;
-        const float2 sh = shift[p];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-          const float d = dp[k * D] * cd;
+          const float d = dv[k] * cd;
           const float2 a = cscale(v[k], cp);
           const float aa = a.x * a.x + a.y * a.y;
           float2 res;
@@ -405,7 +468,6 @@ __global__ __launch_bounds__(WAVE * tile_waves<D>()) void k_tile(
     // overlap-add of c*conj(prb)*frame into the LDS tile, one wave at a time
     // in wave order (fixed summation order)
     const bool adds = active && ps.x >= 0 && ix >= 0 && ix < P;
-    const float2* pp = prb + (long)ix * P - oy;
     const int col = ps.x + ix - x0;
     const int rowb = ps.y - oy - y0;
 #pragma unroll 1
@@ -415,23 +477,25 @@ __global__ __launch_bounds__(WAVE * tile_waves<D>()) void k_tile(
         for (int k = 0; k < D; ++k) {
           const int iy = k - oy;
           if (iy >= 0 && iy < P) {
-            const float2 val = cscale(cmulc(v[k], pp[k]), out_scale);
+            const float2 val = cscale(cmulc(v[k], pr[k]), out_scale);
             float2* dst = acc + (rowb + k) * EB + col;
             atomicAdd(&dst->x, val.x);
             atomicAdd(&dst->y, val.y);
           }
         }
       }
-      __syncthreads();
+      if constexpr (W > 1) __syncthreads();
     }
   }
+  if constexpr (W == 1) wave_sync();
+  else __syncthreads();
   float2* out = scratch + (long)slotid * EB * EB;
   for (int i = threadIdx.x; i < EB * EB; i += W * WAVE) out[i] = acc[i];
 }
 
 // LDS bytes of a k_tile workgroup
 inline size_t tile_lds_bytes(int D, int EB) {
-  const int W = D <= 16 ? 4 : 2, G = 64 / D;
+  const int W = 1, G = 64 / D;
   return sizeof(float2) * (size_t)EB * EB + sizeof(float) * (size_t)W * G * D * (D + 1);
 }
 
@@ -544,8 +608,8 @@ int run_minf(pt_ptycho* h, double* sums, const float2* f, const float2* dir, con
                      h->c, cp, cd, model, lagr, skip);
   }
   PT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_partials, dim3(nchunk), dim3(64), 0, st, sums, h->partial, nb, ngam,
-                     skip);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(nchunk, ngam), dim3(64), 0, st, sums, h->partial, nb,
+                     ngam, skip);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -594,8 +658,9 @@ int tiles_impl(pt_ptycho* h, float2* out, const float2* gin, const float2* f, co
   return run_fold<MODE>(h, out, f, hh, lamd, (float)rho, rho_dev, st);
 }
 
-// LDS budget of an overlap-add workgroup (4 per CU at 40 KB)
-constexpr size_t kTileLdsBudget = 40 * 1024;
+// LDS budget of an overlap-add workgroup (13 resident per CU at 12 KB; for a
+// 16x16 probe the tile is 16 x 16 pixels, extended to 31 x 31)
+constexpr size_t kTileLdsBudget = 12 * 1024;
 
 }  // namespace
 }  // namespace pt

@@ -297,10 +297,14 @@ __global__ __launch_bounds__(64) void k_scatter_own(float2* __restrict__ Bg,
                                                     const int2* __restrict__ wi_range,
                                                     const int* __restrict__ wi_slot, int N,
                                                     int M, float beta, int nt8, int pz) {
-  __shared__ float wxs[64][16];
-  __shared__ float wys[64][16];
-  __shared__ float2 gs[64][kZS];
-  __shared__ int eos[64];
+  // 32 entries per chunk: lanes 0-31 stage the weights of entry `lane`,
+  // lanes 32-63 its slice values; rows padded by one word so the staging
+  // writes fall on different banks (8.8 KB LDS: ~18 workgroups per CU)
+  constexpr int kSE = 32;
+  __shared__ float wxs[kSE][17];
+  __shared__ float wys[kSE][17];
+  __shared__ float2 gs[kSE][kZS + 1];
+  __shared__ int eos[kSE];
   const int twoN = 2 * N;
   const int wi = blockIdx.x, z0 = blockIdx.y * kZS;
   const int lane = threadIdx.x, lx = lane & 7, ly = lane >> 3;
@@ -311,24 +315,28 @@ __global__ __launch_bounds__(64) void k_scatter_own(float2* __restrict__ Bg,
   float2 acc[kZS];
 #pragma unroll
   for (int j = 0; j < kZS; ++j) acc[j] = make_float2(0.f, 0.f);
-  for (int c0 = rg.x; c0 < rg.y; c0 += 64) {
-    const int ne = min(64, rg.y - c0);
+  for (int c0 = rg.x; c0 < rg.y; c0 += kSE) {
+    const int ne = min(kSE, rg.y - c0);
     __syncthreads();
-    if (lane < ne) {
-      const int s = ent_s[c0 + lane];
-      const float2 p = spos[s];
-      const int2 fl = sfloor[s];
+    const int el = lane & (kSE - 1);
+    if (el < ne) {
+      const int s = ent_s[c0 + el];
+      if (lane < kSE) {
+        const float2 p = spos[s];
+        const int2 fl = sfloor[s];
 #pragma unroll
-      for (int k = 0; k < 15; ++k) {
-        const float dx = (float)(fl.x - M + k) * inv2N - p.x;
-        const float dy = (float)(fl.y - M + k) * inv2N - p.y;
-        wxs[lane][k] = (k <= 2 * M) ? __expf(-beta * dx * dx) : 0.f;
-        wys[lane][k] = (k <= 2 * M) ? __expf(-beta * dy * dy) : 0.f;
+        for (int k = 0; k < 15; ++k) {
+          const float dx = (float)(fl.x - M + k) * inv2N - p.x;
+          const float dy = (float)(fl.y - M + k) * inv2N - p.y;
+          wxs[el][k] = (k <= 2 * M) ? __expf(-beta * dx * dx) : 0.f;
+          wys[el][k] = (k <= 2 * M) ? __expf(-beta * dy * dy) : 0.f;
+        }
+        eos[el] = ent_o[c0 + el];
+      } else {
+        const float2* qs = Q + (long)s * pz + z0;
+#pragma unroll
+        for (int j = 0; j < kZS; ++j) gs[el][j] = (z0 + j < pz) ? qs[j] : make_float2(0.f, 0.f);
       }
-      eos[lane] = ent_o[c0 + lane];
-      const float2* qs = Q + (long)s * pz + z0;
-#pragma unroll
-      for (int j = 0; j < kZS; ++j) gs[lane][j] = (z0 + j < pz) ? qs[j] : make_float2(0.f, 0.f);
     }
     __syncthreads();
     for (int e = 0; e < ne; ++e) {
@@ -736,6 +744,16 @@ extern "C" int pt_tomo_set_theta(pt_tomo* h, const float* theta, pt_stream strea
   std::vector<int> fillp(cnt.begin(), cnt.end() - 1), list(ns);
   for (long q = 0; q < ns; ++q)
     list[fillp[(long)((fl[q].y + N) / kTile) * nt1 + (fl[q].x + N) / kTile]++] = (int)q;
+  // within a bin, samples in row-major order of their floor position: the
+  // lanes of a wavefront then read neighbouring LDS words for each tap (few
+  // bank conflicts); each lane still computes its own sample, so the order
+  // does not change any result
+  for (long tI = 0; tI < ntiles; ++tI)
+    std::sort(list.begin() + cnt[tI], list.begin() + cnt[tI + 1], [&](int a, int b) {
+      if (fl[a].y != fl[b].y) return fl[a].y < fl[b].y;
+      if (fl[a].x != fl[b].x) return fl[a].x < fl[b].x;
+      return a < b;
+    });
   std::vector<int> wtile;
   std::vector<int2> wrange;
   for (long tI = 0; tI < ntiles; ++tI)

@@ -97,23 +97,28 @@ __global__ __launch_bounds__(RB) void k_seg_reduce(double* __restrict__ partial,
   }
 }
 
-__global__ void k_finish(double* __restrict__ out, const double* __restrict__ partial, int nblk,
-                         int nv) {
-  const int seg = blockIdx.x, k = threadIdx.x;
-  if (k >= nv) return;
+// second pass: one wave per (segment, accumulator): lane-strided sums in block
+// order, then a fixed-shape wave tree (deterministic)
+__global__ __launch_bounds__(64) void k_finish(double* __restrict__ out,
+                                               const double* __restrict__ partial, int nblk,
+                                               int nv) {
+  const int seg = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partial[((long)seg * nblk + b) * nv + k];
-  out[seg * nv + k] = s;
+  for (int b = lane; b < nblk; b += 64) s += partial[((long)seg * nblk + b) * nv + k];
+  s = wave_sum(s);
+  if (lane == 0) out[seg * nv + k] = s;
 }
 
-__global__ void k_finish_n(double* __restrict__ out, const double* __restrict__ partial,
-                           int nblk, int nv, int nout, const double* __restrict__ skip) {
-  const int seg = blockIdx.x, k = threadIdx.x;
-  if (k >= nout) return;
+__global__ __launch_bounds__(64) void k_finish_n(double* __restrict__ out,
+                                                 const double* __restrict__ partial, int nblk,
+                                                 int nv, int nout,
+                                                 const double* __restrict__ skip) {
+  const int seg = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
   if (skip && skip[2 * seg + 1] != 0.0) return;
   double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partial[((long)seg * nblk + b) * nv + k];
-  out[seg * nout + k] = s;
+  for (int b = lane; b < nblk; b += 64) s += partial[((long)seg * nblk + b) * nv + k];
+  s = wave_sum(s);
+  if (lane == 0) out[seg * nout + k] = s;
 }
 
 template <int NV, typename F>
@@ -128,7 +133,7 @@ int seg_reduce(double* out, long len, int nseg, F f, hipStream_t st) {
   hipLaunchKernelGGL((k_seg_reduce<NV, F>), dim3(nblk, nseg), dim3(RB), 0, st, part, seglen, f,
                      (const double*)nullptr);
   PT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_finish, dim3(nseg), dim3(64), 0, st, out, part, nblk, NV);
+  hipLaunchKernelGGL(k_finish, dim3(nseg, NV), dim3(64), 0, st, out, part, nblk, NV);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -149,7 +154,8 @@ int seg_reduce_n(double* out, long len, int nseg, int nout, F f, hipStream_t st,
   hipLaunchKernelGGL((k_seg_reduce<NV, F>), dim3(nblk, nseg), dim3(RB), 0, st, part, seglen, f,
                      skip);
   PT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_finish_n, dim3(nseg), dim3(64), 0, st, out, part, nblk, NV, nout, skip);
+  hipLaunchKernelGGL(k_finish_n, dim3(nseg, nout), dim3(64), 0, st, out, part, nblk, NV, nout,
+                     skip);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
