@@ -136,6 +136,10 @@ def main():
                     help="only run 5 device copies of MIB MiB (HBM counter calibration)")
     ap.add_argument("--tol-run", type=int, default=0,
                     help="also run N plain iterations for time-to-Lagrangian-tolerance")
+    ap.add_argument("--warm-weights", default=None,
+                    help="U-Net weights: time plain ADMM and the learned warm start to the "
+                         "plain method's converged Lagrangian (--tol-iters iterations)")
+    ap.add_argument("--tol-iters", type=int, default=100)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -236,6 +240,14 @@ def main():
         out["lagr_last"] = lag[-1]
     if args.tol_run:
         out["time_to_tol"] = time_to_tol(slv, data, w, run, args.tol_run)
+    if args.warm_weights:
+        from ptychotomo_amd import warmstart
+        log(f"warm start: plain vs U-Net, {args.tol_iters} iterations each")
+        del slv, data
+        torch.cuda.empty_cache()
+        net = warmstart.load(args.warm_weights)
+        out["time_to_tol"] = warmstart.compare(net, w, n_ref=args.tol_iters,
+                                               n_max=args.tol_iters, tol=1e-3)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(w)

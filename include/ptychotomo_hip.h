@@ -129,10 +129,13 @@ int pt_ptycho_grad(pt_ptycho* h, void* grad, const void* psi, const float* data,
                    const void* lamd, double rho, const double* rho_dev, float coefptycho,
                    float coefdata, int model, pt_stream stream);
 /* Data term of the cg_ptycho functional (solver.py:208-212) at ngam step
- * candidates psi + gamma*d, segmented by angle chunks of `chunk` angles:
- *   sums[c*ngam + j] = sum_{chunk c} |a|^2 - 2 d mlog|a|   (poisson)
- *                    = sum_{chunk c} (|a| - sqrt(d))^2      (gaussian)
- *   a = coefptycho*F(psi + gammas[c*ngam+j]*dir).
+ * candidates psi + gamma*d, segmented by angle chunks of `chunk` angles, as
+ * its change from gamma = 0 (the quantity the halving rule decides on):
+ *   sums[c*ngam + j] = sum_{chunk c} t(a_j) - t(a_0),
+ *   t(a) = |a|^2 - 2 d mlog|a|  (poisson),  (|a| - sqrt(d))^2  (gaussian),
+ *   a_j = coefptycho*F(psi) + gammas[c*ngam+j]*coefptycho*F(dir) formed in FP32,
+ * evaluated per element (difference of squares as a product, log ratio as
+ * log1p) and summed in FP64.
  * sums, gammas: device FP64 arrays.  dir may be NULL (then only gamma = 0
  * is meaningful).  With lagr != 0 the poisson sum also subtracts
  * d - 2 d mlog(sqrt d), giving lagr[0] of take_lagr (solver.py:294-296).
@@ -214,20 +217,21 @@ int pt_k_tomo_resid(void* r, double* out, const void* K, const void* Ru, const v
  * out[0] = ||J u - xi1||^2.  u: [nz,ny,nx], xi1: [3,nz,ny,nx]. */
 int pt_k_reg_grad(void* grad, double* out, const void* u, const void* xi1, double tau,
                   const double* tau_dev, int nz, int ny, int nx, pt_stream stream);
-/* Line-search candidates (solver.py:171-177), evaluated as the reference
- * forms them: the stepped quantity x + gamma*dx element-wise in FP32, sums in
- * FP64, so a step below FP32 resolution reproduces the gamma = 0 value
- * exactly.  gammas: device FP64, ngam (<= 8) candidates per segment.
- * cg_ptycho rho term (solver.py:213), per angle-chunk segment:
- *   out[s*ngam+j] = ||h - (psi + g_j d) + lamd/rho||^2 */
+/* Line-search candidates (solver.py:171-177), formed as the reference forms
+ * them: the stepped quantity x + gamma*dx element-wise in FP32, so a step
+ * below FP32 resolution reproduces the gamma = 0 value exactly.  Each sum is
+ * the change from gamma = 0, |r_g|^2 - |r_0|^2 = (r_g - r_0).(r_g + r_0) per
+ * element, summed in FP64.  gammas: device FP64, ngam (<= 8) candidates per
+ * segment.  cg_ptycho rho term (solver.py:213), per angle-chunk segment:
+ *   out[s*ngam+j] = ||h - (psi + g_j d) + lamd/rho||^2 - (g_j = 0 value) */
 int pt_k_rho_minf(double* out, const void* h, const void* psi, const void* lamd, const void* d,
                   double rho, const double* rho_dev, const double* gammas, int ngam, long len,
                   int nseg, const double* skip, pt_stream stream);
-/* cg_tomo (solver.py:182-183): out[j] = ||K*Ru + g_j*(K*Rd) - xi0||^2 */
+/* cg_tomo (solver.py:182-183): out[j] = change of ||K*Ru + g_j*(K*Rd) - xi0||^2 */
 int pt_k_tomo_minf(double* out, const void* K, const void* Ru, const void* Rd, const void* xi0,
                    const double* gammas, int ngam, long len, const double* skip,
                    pt_stream stream);
-/* cg_tomo: out[j] = ||J u + g_j*J d - xi1||^2 */
+/* cg_tomo: out[j] = change of ||J u + g_j*J d - xi1||^2 */
 int pt_k_reg_minf(double* out, const void* u, const void* d, const void* xi1,
                   const double* gammas, int ngam, int nz, int ny, int nx, const double* skip,
                   pt_stream stream);

@@ -220,44 +220,88 @@ __global__ __launch_bounds__(WAVE) void k_fwd(float2* __restrict__ g, const floa
 }
 
 // ------------------------------------------------------------------ minf
-// log|a| for |a|^2 = s, with |a| clamped below at 1e-32 as mlog does
-// (solver.py:48-52); accurate for every s (used where s < FLT_MIN).
-__device__ __forceinline__ float log_abs_exact(float2 a) {
-  return logf(fmaxf(sqrtf(a.x * a.x + a.y * a.y), 1e-32f));
+// log|a| with |a| clamped below at 1e-32 as mlog does (solver.py:48-52), for
+// s = |a|^2: the hardware log of s where s is a normal float; below that a
+// is scaled by 2^64 first (|a| >= 1e-32 then gives a normal square).
+__device__ __forceinline__ float log_abs(float2 a, float s) {
+  if (s >= 1.17549435e-38f) return 0.5f * __logf(s);
+  const float x = a.x * 1.8446744e19f, y = a.y * 1.8446744e19f;
+  return fmaxf(0.5f * __logf(x * x + y * y) - 44.3614196f, -73.6827235f);
 }
 
-// Poisson data term of one lane's far-field column at step gj: the hardware
-// log for |a|^2 >= FLT_MIN, the accurate path for the (rare) rest.
-template <int D>
-__device__ __forceinline__ float poisson_part(const float2 (&a)[D], const float2 (&b)[D],
-                                              const float (&d)[D], float gj) {
+// log(1 + x): six-term series for |x| < 1/16 (truncation below 4e-10
+// relative), the hardware log of 1 + x otherwise (1 + x then exact to 6e-8
+// relative of a log of at least 0.06)
+__device__ __forceinline__ float log1p_step(float x) {
+  if (fabsf(x) < 0.0625f)
+    return x * (1.f + x * (-0.5f + x * (0.33333334f + x * (-0.25f + x * (0.2f + x * -0.16666667f)))));
+  return __logf(1.f + x);
+}
+
+// Line-search decision quantity of one lane's far-field column: the change
+// f(gj) - f(0) of the data term, sum_k, with ag = a + gj*b formed in FP32 as
+// the reference forms fpsi + gamma*fd (solver.py:172; a step below FP32
+// resolution gives exactly 0).  Evaluated per element -- |ag|^2 - |a|^2 as a
+// product of the (exact) difference and the sum, the log ratio as log1p --
+// instead of as a difference of two large sums, whose FP32 rounding is of
+// the size of the differences decided on near convergence.
+//   poisson:  |ag|^2 - |a|^2 - 2 d (log|ag| - log|a|)
+//   gaussian: (|ag| - sqrt d)^2 - (|a| - sqrt d)^2   (d holds sqrt d)
+template <int D, int MODEL>
+__device__ __forceinline__ float delta_part(const float2 (&a)[D], const float2 (&b)[D],
+                                            const float (&d)[D], float gj) {
   float part = 0.f;
-  bool rare = false;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     const float2 ag = make_float2(a[k].x + gj * b[k].x, a[k].y + gj * b[k].y);
-    const float s2 = ag.x * ag.x + ag.y * ag.y;
-    rare |= !(s2 >= 1.17549435e-38f);
-    part += s2 - d[k] * __logf(s2);  // 2 d log|a| = d log|a|^2
-  }
-  if (rare) {
-    part = 0.f;
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      const float2 ag = make_float2(a[k].x + gj * b[k].x, a[k].y + gj * b[k].y);
-      part += ag.x * ag.x + ag.y * ag.y - 2.f * d[k] * log_abs_exact(ag);
+    const float dx = ag.x - a[k].x, dy = ag.y - a[k].y;
+    const float ds = dx * (ag.x + a[k].x) + dy * (ag.y + a[k].y);
+    const float s0 = a[k].x * a[k].x + a[k].y * a[k].y;
+    const float sg = ag.x * ag.x + ag.y * ag.y;
+    if constexpr (MODEL == PT_MODEL_POISSON) {
+      // log(|ag|^2 / |a|^2); where either is below FLT_MIN, mlog's clamp
+      // at 1e-32 decides and the logs are taken separately
+      const float lr = (s0 >= 1.17549435e-38f && sg >= 1.17549435e-38f)
+                           ? log1p_step(ds * __builtin_amdgcn_rcpf(s0))
+                           : 2.f * (log_abs(ag, sg) - log_abs(a[k], s0));
+      part += ds - d[k] * lr;
+    } else {
+      const float m0 = sqrtf(s0), mg = sqrtf(sg);
+      const float den = m0 + mg;
+      const float dm = den > 0.f ? ds / den : 0.f;
+      part += ds - 2.f * d[k] * dm;
     }
   }
   return part;
 }
 
-// Sums over patches of the data term at ngam step candidates; one partial
-// per (chunk, block) in partial[(c*gridDim.x + b)*ngam + j].  The candidate
+// Absolute data term at gamma = 0 (take_lagr's lagr[0], solver.py:294-296)
+template <int D>
+__device__ __forceinline__ float data_term(const float2 (&a)[D], const float (&d)[D], int model) {
+  float part = 0.f;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const float s2 = a[k].x * a[k].x + a[k].y * a[k].y;
+    if (model == PT_MODEL_POISSON) {
+      part += s2 - 2.f * d[k] * log_abs(a[k], s2);
+    } else {
+      const float e = sqrtf(s2) - d[k];
+      part += e * e;
+    }
+  }
+  return part;
+}
+
+// Sums over patches, per chunk, of the data term's change f(gamma_j) - f(0)
+// at ngam step candidates (lagr = 0), or of the absolute data term of
+// take_lagr (lagr != 0, gamma = 0 only); one partial per (chunk, block) in
+// partial[(c*gridDim.x + b)*ngam + j].  The candidate
 // loop is not unrolled and its FP64 accumulators rotate in registers; the
 // LDS footprint is the transpose buffer only.
 #define PT_MAX_GAM 8
-template <int D>
-__global__ __launch_bounds__(WAVE) void k_minf(double* __restrict__ partial,
+template <int D, int MODEL>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3)))
+void k_minf(double* __restrict__ partial,
                                                const float2* __restrict__ f,
                                                const float2* __restrict__ dir,
                                                const float* __restrict__ data,
@@ -323,22 +367,16 @@ __global__ __launch_bounds__(WAVE) void k_minf(double* __restrict__ partial,
     // below FP32 resolution leaves the term bit-identical to gamma = 0.
     // Candidate j accumulates into acc[0]; the accumulators rotate once per
     // candidate slot (8 rotations per patch group restore their order).
+    if (lagr) {
+      acc[0] += (double)(data_term<D>(a, d, model) - dterm);
+      continue;
+    }
 #pragma unroll 1
     for (int j = 0; j < PT_MAX_GAM; ++j) {
-      if (j < ngam) {
+      // gamma = 0 (candidate 0 of a sweep) changes nothing: its sum stays 0
+      if (j < ngam && gsh[j] != 0.f) {
         const float gj = gsh[j];
-        float part;
-        if (model == PT_MODEL_POISSON) {
-          part = poisson_part<D>(a, b, d, gj) - dterm;
-        } else {
-          part = 0.f;
-#pragma unroll
-          for (int k = 0; k < D; ++k) {
-            const float2 ag = make_float2(a[k].x + gj * b[k].x, a[k].y + gj * b[k].y);
-            const float e = sqrtf(ag.x * ag.x + ag.y * ag.y) - d[k];
-            part += e * e;
-          }
-        }
+        const float part = delta_part<D, MODEL>(a, b, d, gj);
         acc[0] += (double)part;
       }
       const double t0 = acc[0];
@@ -405,8 +443,17 @@ __global__ __launch_bounds__(WAVE * tile_waves<D>()) void k_tile(
   __syncthreads();
   const int beg = tile_start[tile], end = tile_start[tile + 1];
   const int ix = r - ox;
-  float2 pr[D];   // c * prb column of this lane's frame row (overlap-add uses conj)
-  probe_row<D>(pr, prb, r, P, ox, oy, 1.0f);
+  // probe column of this lane's frame row in registers (frames up to 32 x 32;
+  // 64 x 64 frames read it from memory, which is cached)
+  constexpr int NPR = D <= 32 ? D : 1;
+  float2 pr[NPR];
+  if constexpr (D <= 32) probe_row<D>(pr, prb, r, P, ox, oy, 1.0f);
+  auto prv = [&](int k) -> float2 {
+    if constexpr (D <= 32) return pr[k];
+    const int iy = k - oy;
+    return (ix >= 0 && ix < P && iy >= 0 && iy < P) ? prb[(long)ix * P + iy]
+                                                    : make_float2(0.f, 0.f);
+  };
   for (int i0 = beg; i0 < end; i0 += W * G) {
     const int ii = i0 + wave * G + slot;
     const bool active = ii < end;
@@ -435,7 +482,7 @@ __global__ __launch_bounds__(WAVE * tile_waves<D>()) void k_tile(
       }
       const float2 sh = active ? shift[p] : make_float2(1.f, 0.f);
 #pragma unroll
-      for (int k = 0; k < D; ++k) v[k] = cscale(cmul(pr[k], v[k]), cs);
+      for (int k = 0; k < D; ++k) v[k] = cscale(cmul(prv(k), v[k]), cs);
       frame_fwd_w<D>(v, T, slot, r);
       // residual in the unshifted frame (the per-patch unimodular factor of
       // kernels.cuh:319-355 cancels: r(s*a) = s*r(a), F* removes s again)
@@ -465,23 +512,40 @@ __global__ __launch_bounds__(WAVE * tile_waves<D>()) void k_tile(
       }
     }
     frame_inv_w<D>(v, T, slot, r);
-    // overlap-add of c*conj(prb)*frame into the LDS tile, one wave at a time
-    // in wave order (fixed summation order)
+    // overlap-add of c*conj(prb)*frame into the LDS tile.  The G patches of a
+    // wave overlap, so they are added one slot at a time, in slot order
+    // (deterministic): within a slot every lane owns distinct pixels, so a
+    // plain LDS read-add-write (all 16 rows read first, then written) needs
+    // no atomics; waves of a workgroup take turns.
     const bool adds = active && ps.x >= 0 && ix >= 0 && ix < P;
     const int col = ps.x + ix - x0;
     const int rowb = ps.y - oy - y0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) v[k] = cscale(cmulc(v[k], prv(k)), out_scale);
+    constexpr int KB = D < 8 ? D : 8;   // rows read before they are written back
 #pragma unroll 1
     for (int w = 0; w < W; ++w) {
-      if (w == wave && adds) {
+      if (w == wave) {
+#pragma unroll 1
+        for (int sl = 0; sl < G; ++sl) {
+          if (sl == slot && adds) {
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-          const int iy = k - oy;
-          if (iy >= 0 && iy < P) {
-            const float2 val = cscale(cmulc(v[k], pr[k]), out_scale);
-            float2* dst = acc + (rowb + k) * EB + col;
-            atomicAdd(&dst->x, val.x);
-            atomicAdd(&dst->y, val.y);
+            for (int k0 = 0; k0 < D; k0 += KB) {
+              float2 cur[KB];
+#pragma unroll
+              for (int kk = 0; kk < KB; ++kk) {
+                const int iy = k0 + kk - oy;
+                if (iy >= 0 && iy < P) cur[kk] = acc[(rowb + k0 + kk) * EB + col];
+              }
+#pragma unroll
+              for (int kk = 0; kk < KB; ++kk) {
+                const int iy = k0 + kk - oy;
+                if (iy >= 0 && iy < P)
+                  acc[(rowb + k0 + kk) * EB + col] = cadd(cur[kk], v[k0 + kk]);
+              }
+            }
           }
+          wave_sync();
         }
       }
       if constexpr (W > 1) __syncthreads();
@@ -603,9 +667,14 @@ int run_minf(pt_ptycho* h, double* sums, const float2* f, const float2* dir, con
   }
   {
   ProfScope prof("ptycho_minf", st);
-  hipLaunchKernelGGL(k_minf<D>, dim3(nb, nchunk), dim3(WAVE), 0, st, h->partial, f, dir, data,
-                     h->prb, h->pos, gammas, ngam, chunk, h->nz, h->n, h->S, h->P, h->ox, h->oy,
-                     h->c, cp, cd, model, lagr, skip);
+  if (model == PT_MODEL_POISSON)
+    hipLaunchKernelGGL((k_minf<D, PT_MODEL_POISSON>), dim3(nb, nchunk), dim3(WAVE), 0, st,
+                       h->partial, f, dir, data, h->prb, h->pos, gammas, ngam, chunk, h->nz, h->n,
+                       h->S, h->P, h->ox, h->oy, h->c, cp, cd, model, lagr, skip);
+  else
+    hipLaunchKernelGGL((k_minf<D, PT_MODEL_GAUSSIAN>), dim3(nb, nchunk), dim3(WAVE), 0, st,
+                       h->partial, f, dir, data, h->prb, h->pos, gammas, ngam, chunk, h->nz, h->n,
+                       h->S, h->P, h->ox, h->oy, h->c, cp, cd, model, lagr, skip);
   }
   PT_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_reduce_partials, dim3(nchunk, ngam), dim3(64), 0, st, sums, h->partial, nb,

@@ -451,11 +451,14 @@ struct FRegGrad {
     grad[i] = make_float2(gv.x + c * s.x, gv.y + c * s.y);
   }
 };
-// Line-search candidates, evaluated as the reference forms them: the
-// stepped quantity x + gamma*dx element-wise in FP32 (a step below FP32
-// resolution gives a bit-identical term), accumulation in FP64.
+// Line-search candidates, formed as the reference forms them: the stepped
+// quantity x + gamma*dx element-wise in FP32 (a step below FP32 resolution
+// gives exactly the gamma = 0 value).  What is summed (in FP64) is the change
+// of the squared residual from gamma = 0, per element, |r_g|^2 - |r_0|^2 =
+// (r_g - r_0).(r_g + r_0): the halving rule decides on f(gamma) - f(0), and
+// this form does not lose it to the rounding of two large sums.
 #define PT_MAX_CAND 8
-// cg_ptycho rho term (solver.py:213): |h - (psi + g*d) + lamd/rho|^2
+// cg_ptycho rho term (solver.py:213): change of |h - (psi + g*d) + lamd/rho|^2
 struct FRhoMinf {
   const float2* h; const float2* psi; const float2* lamd; const float2* d; SV rhov;
   const double* gam; int ngam; long seglen;
@@ -464,35 +467,37 @@ struct FRhoMinf {
     const float rho = (float)rhov.get();
     const float2 a = h[i], p = psi[i], l = lamd[i], dd = d[i];
     const float lx = l.x / rho, ly = l.y / rho;
+    const float r0x = a.x - p.x + lx, r0y = a.y - p.y + ly;
 #pragma unroll
     for (int j = 0; j < PT_MAX_CAND; ++j) {
       if (j < ngam) {
         const float g = (float)gam[s * ngam + j];
         const float tx = p.x + g * dd.x, ty = p.y + g * dd.y;
         const float rx = a.x - tx + lx, ry = a.y - ty + ly;
-        v[j] += (double)(rx * rx + ry * ry);
+        v[j] += (double)((rx - r0x) * (rx + r0x) + (ry - r0y) * (ry + r0y));
       }
     }
   }
 };
-// cg_tomo data term (solver.py:182-183): |K*Ru + g*(K*Rd) - xi0|^2
+// cg_tomo data term (solver.py:182-183): change of |K*Ru + g*(K*Rd) - xi0|^2
 struct FTomoMinf {
   const float2* K; const float2* Ru; const float2* Rd; const float2* xi0; const double* gam;
   int ngam;
   __device__ void operator()(long i, double (&v)[PT_MAX_CAND]) const {
     const float2 k = K[i];
     const float2 a = cmul(k, Ru[i]), b = cmul(k, Rd[i]), x = xi0[i];
+    const float r0x = a.x - x.x, r0y = a.y - x.y;
 #pragma unroll
     for (int j = 0; j < PT_MAX_CAND; ++j) {
       if (j < ngam) {
         const float g = (float)gam[j];
         const float rx = (a.x + g * b.x) - x.x, ry = (a.y + g * b.y) - x.y;
-        v[j] += (double)(rx * rx + ry * ry);
+        v[j] += (double)((rx - r0x) * (rx + r0x) + (ry - r0y) * (ry + r0y));
       }
     }
   }
 };
-// cg_tomo regulariser term: |J u + g*J d - xi1|^2 over the 3 components
+// cg_tomo regulariser term: change of |J u + g*J d - xi1|^2 over the 3 components
 struct FRegMinf {
   const float2* u; const float2* d; const float2* xi1; const double* gam; int ngam;
   int nz, ny, nx;
@@ -514,7 +519,8 @@ struct FRegMinf {
         for (int k = 0; k < 3; ++k) {
           const float rx = (gu[k].x + g * gd[k].x) - xi[k].x;
           const float ry = (gu[k].y + g * gd[k].y) - xi[k].y;
-          dacc += (double)(rx * rx + ry * ry);
+          const float r0x = gu[k].x - xi[k].x, r0y = gu[k].y - xi[k].y;
+          dacc += (double)((rx - r0x) * (rx + r0x) + (ry - r0y) * (ry + r0y));
         }
         v[j] += dacc;
       }

@@ -41,8 +41,10 @@ from .radonusfft import radonusfft
 PLANCK_CONSTANT = 6.58211928e-19  # [keV*s]   solver.py:15
 SPEED_OF_LIGHT = 299792458e+2     # [cm/s]    solver.py:16
 
-_MAXGAM = 8      # line-search candidates per sweep (gamma = 0 and 7 halvings)
-_SWEEPS = 6      # sweeps per line search: 42 halvings take 2 below 1e-12
+_MAXGAM = 8      # cg_tomo line-search candidates per sweep (gamma = 0 and 7 halvings)
+_SWEEPS = 6      # its sweeps per line search: 42 halvings take 2 below 1e-12
+_PGAM = 5        # cg_ptycho: gamma = 0 and 4 halvings per sweep (one sweep covers the
+_PSWEEPS = 11    # usual 0-3 halvings; 44 halvings take 2 below 1e-12)
 # device scalar block (FP64): 1, rho, tau, takexi {pshift, cos, sin, kfac},
 # takexi statistics, penalty norms, max |angle psi|
 _ONE, _RHO, _TAU, _PRM, _STATS, _NORMS, _MAXANG, _NSC = 0, 1, 2, 3, 8, 12, 16, 24
@@ -304,19 +306,19 @@ class Solver(object):
             gamma = 0
         return gamma
 
-    def _line_search_dev(self, w, nseg, evalf, weights):
+    def _line_search_dev(self, w, nseg, evalf, weights, ngam=_MAXGAM, sweeps=_SWEEPS):
         """The halving rule of line_search for nseg independent segments, on
-        the device: sweeps of 7 halvings (plus gamma = 0) until every segment
-        has accepted a step (_SWEEPS sweeps cover 2 -> 1e-12).  evalf(cand,
-        state, f1, f2) enqueues the two functional terms at the candidates,
-        skipping finished segments; weights points to their two FP64
-        weights."""
+        the device: sweeps of ngam - 1 halvings (plus gamma = 0) until every
+        segment has accepted a step (`sweeps` sweeps cover 2 -> 1e-12).
+        evalf(cand, state, f1, f2) enqueues the two functional terms (their
+        change from gamma = 0) at the candidates, skipping finished segments;
+        weights points to their two FP64 weights."""
         st = stream()
-        _call("pt_k_ls_begin", ptr(w["state"]), ptr(w["cand"]), nseg, _MAXGAM, st)
-        for sw in range(_SWEEPS):
+        _call("pt_k_ls_begin", ptr(w["state"]), ptr(w["cand"]), nseg, ngam, st)
+        for sw in range(sweeps):
             evalf(w["cand"], w["state"], w["f1"], w["f2"])
             _call("pt_k_ls_select", ptr(w["state"]), ptr(w["cand"]), ptr(w["step"]),
-                  ptr(w["f1"]), ptr(w["f2"]), weights, nseg, _MAXGAM, int(sw == _SWEEPS - 1), st)
+                  ptr(w["f1"]), ptr(w["f2"]), weights, nseg, ngam, int(sw == sweeps - 1), st)
 
     def _dai_yuan_dev(self, w, i, grad, grad0, L, nseg):
         """d = -grad (i = 0) or -grad + beta d (solver.py:192-196, 227-231)."""
@@ -430,12 +432,12 @@ class Solver(object):
 
             def evalf(cand, state, f1, f2):
                 _call("pt_ptycho_minf", op._h, ptr(f1), ptr(psi), ptr(d), ptr(data), ptr(cand),
-                      _MAXGAM, chunk, cp, cd, mdl, 0, ptr(state), st)
+                      _PGAM, chunk, cp, cd, mdl, 0, ptr(state), st)
                 _call("pt_k_rho_minf", ptr(f2), ptr(h), ptr(psi), ptr(lamd), ptr(d), 0.0,
-                      _dp(blk, _RHO), ptr(cand), _MAXGAM, L, nseg, ptr(state), st)
+                      _dp(blk, _RHO), ptr(cand), _PGAM, L, nseg, ptr(state), st)
 
             # minf = data term + rho*|h - psi + lamd/rho|^2 (solver.py:208-213)
-            self._line_search_dev(w, nseg, evalf, _dp(blk, _ONE))
+            self._line_search_dev(w, nseg, evalf, _dp(blk, _ONE), _PGAM, _PSWEEPS)
             _call("pt_k_axpy_seg", ptr(psi), ptr(w["step"]), ptr(d), L, nseg, st)
             if hist is not None:
                 hist[i].copy_(w["step"][0::2])

@@ -195,17 +195,32 @@ def test_admm_trajectory(setup):
     uc, pc = r64.admm(data, init["h"], init["e"], init["psi"], init["phi"], init["lamd"],
                       init["mu"], init["u"], 3e-7, 4, 4, niter, "poisson", lagr_every=1)
     assert len(slv.history) == len(r64.history) == niter
+
+    def steps(h):
+        return [g for r in h for ch in r["ptycho_gammas"] for g in ch] + \
+            [g for r in h for g in r["tomo_gammas"]]
+    # halving decisions that differ from the FP64 twin: reported, not required
+    # to be zero -- the first two iterations run 36 CG iterations each, where a
+    # functional difference of FP32 rounding size can flip one halving
+    ga, gb, gc = steps(slv.history), steps(ref.history), steps(r64.history)
+    flips_gpu = sum(x != z for x, z in zip(ga, gc))
+    flips_f32 = sum(y != z for y, z in zip(gb, gc))
+    print(f"step decisions differing from FP64: gpu {flips_gpu}, fp32 restatement {flips_f32} "
+          f"of {len(gc)}")
+    assert flips_gpu <= max(2, 2 * flips_f32)
+    # the parity bar: Lagrangian trajectory and the reconstruction within 1e-4
+    # relative of the restatement (and of its FP64 twin)
     for ra, rb, rc in zip(slv.history, ref.history, r64.history):
         print(ra["m"], ra["rho"], rb["rho"], ra["tau"], rb["tau"], ra["lagr"][6], rb["lagr"][6],
               rc["lagr"][6])
         assert ra["rho"] == rc["rho"] and ra["tau"] == rc["tau"]
-        assert ra["tomo_gammas"] == rc["tomo_gammas"]
-        assert ra["ptycho_gammas"] == rc["ptycho_gammas"]
         la, lb, lc = ra["lagr"][6], rb["lagr"][6], rc["lagr"][6]
-        assert abs(la - lc) <= max(1.5 * abs(lb - lc), 1e-4 * abs(lc)), (la, lb, lc)
+        assert abs(la - lb) <= 1e-4 * abs(lb) and abs(la - lc) <= 1e-4 * abs(lc), (la, lb, lc)
     eu, ep = rel(ua, uc), rel(pa, pc)
     du, dp = rel(ub, uc), rel(pb, pc)
-    print(f"u: gpu-fp64 {eu:.2e} fp32-fp64 {du:.2e}; psi: gpu-fp64 {ep:.2e} fp32-fp64 {dp:.2e}")
+    print(f"u: gpu-fp64 {eu:.2e} fp32-fp64 {du:.2e} gpu-fp32 {rel(ua, ub):.2e}; "
+          f"psi: gpu-fp64 {ep:.2e} fp32-fp64 {dp:.2e} gpu-fp32 {rel(pa, pb):.2e}")
+    assert rel(ua, ub) <= 1e-4 and rel(pa, pb) <= 1e-4
     assert eu <= max(1.5 * du, 1e-4) and ep <= max(1.5 * dp, 1e-4)
 
 
@@ -261,15 +276,22 @@ def test_ptycho_grad_minf(setup, model):
         # small: the bar is the FP32 restatement's own distance to FP64
         assert e64 <= max(1.5 * e3264, 1e-5), (e64, e3264)
         fd = ref.fwd_ptycho(dirn[ids])
-        for j, gm in enumerate(gam):
+        d64 = datap.astype(np.float64)
+
+        def fval(gm):
             a = (fpsi + np.float32(gm) * fd).astype(np.complex128)
-            d64 = datap.astype(np.float64)
             if model == "poisson":
-                fr = np.sum(np.abs(a) ** 2 - 2 * d64 * np.log(np.maximum(np.abs(a), 1e-32)))
-            else:
-                fr = np.sum((np.abs(a) - np.sqrt(d64)) ** 2)
-            print("chunk", k, "gamma", gm, s[k, j], fr)
-            assert abs(s[k, j] - fr) <= 1e-5 * abs(fr) + 1e-3
+                return np.sum(np.abs(a) ** 2 - 2 * d64 * np.log(np.maximum(np.abs(a), 1e-32)))
+            return np.sum((np.abs(a) - np.sqrt(d64)) ** 2)
+        f0 = fval(0.0)
+        # the kernel returns the change f(gamma) - f(0) (what the halving rule
+        # decides on), exactly 0 at gamma = 0
+        for j, gm in enumerate(gam):
+            df = fval(gm) - f0
+            print("chunk", k, "gamma", gm, s[k, j], df)
+            if gm == 0:
+                assert s[k, j] == 0.0
+            assert abs(s[k, j] - df) <= 1e-5 * abs(f0) + 1e-3
 
 
 def test_admm_graph_replay_matches_eager(setup):
