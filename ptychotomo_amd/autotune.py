@@ -3,7 +3,8 @@ reference).
 
 For a tomography handle it times the forward and adjoint operators over the
 launch shapes the LDS FFTs admit -- FFT lines per workgroup of the row pass,
-the column pass and the adjoint detector pass -- one parameter at a time
+the column passes and the detector-line passes (the forward's inverse DFT
+and the adjoint's DFT) -- one parameter at a time
 (each belongs to its own kernel), keeps the fastest, and caches the result
 per (n, ntheta, pnz) in a JSON file.  The shapes change only the launch, not
 the arithmetic: results are bit-identical across shapes.
@@ -64,9 +65,10 @@ def tune(op, nz, n, ntheta, cache=CACHE, log=None):
     best = list(get(op))
     timings = {"default": list(best)}
     # one parameter at a time: rows (row pass, forward), cols (column
-    # passes, both), adjoint lines (adjoint detector pass)
-    for idx, cands, fn in ((0, [1, 2, 3, 4, 6, 8], fwd), (1, [1, 2, 4, 8], lambda: (fwd(), adj())),
-                           (2, [1, 2, 4, 8], adj)):
+    # passes, both), detector lines (line passes, both)
+    both = lambda: (fwd(), adj())
+    for idx, cands, fn in ((0, [1, 2, 3, 4, 6, 8], fwd), (1, [1, 2, 4, 8], both),
+                           (2, [1, 2, 4, 8], both)):
         res = {}
         for v in cands:
             shape = list(best)

@@ -108,17 +108,17 @@ __global__ __launch_bounds__(256) void k_row_fwd(float2* __restrict__ A, const f
     // centred -> rotated index a' = (c + N/2) mod N
     int a = c + N / 2;
     if (a >= N) a -= N;
-    lds[line * N + a] = val;
+    lds[line * (N + 1) + a] = val;
   }
   __syncthreads();
-  lds_fft_lines_t<-1>(lds, N, nl, plan, tw, threadIdx.x, blockDim.x);
+  lds_fft_lines_t<-1>(lds, N + 1, nl, plan, tw, threadIdx.x, blockDim.x);
   for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
     const int line = i / N, b = i - line * N;
     const int r = line >> 1, q = q0 + r;
     if (q >= N) continue;
     const int m = b < N / 2 ? b : b - N;  // centred frequency
     const int kx = 2 * m + N + (line & 1);
-    A[((long)z * N + q) * (2 * N) + kx] = lds[line * N + b];
+    A[((long)z * N + q) * (2 * N) + kx] = lds[line * (N + 1) + b];
   }
 }
 
@@ -136,18 +136,18 @@ __global__ __launch_bounds__(256) void k_col_fwd(float2* __restrict__ Bg, const 
     const float2 val = A[((long)z * N + q) * (2 * N) + k0 + cc];
     int a = q + N / 2;
     if (a >= N) a -= N;
-    lds[(2 * cc) * N + a] = val;
-    lds[(2 * cc + 1) * N + a] = cmul(val, twist[q]);
+    lds[(2 * cc) * (N + 1) + a] = val;
+    lds[(2 * cc + 1) * (N + 1) + a] = cmul(val, twist[q]);
   }
   __syncthreads();
-  lds_fft_lines_t<-1>(lds, N, nl, plan, tw, threadIdx.x, blockDim.x);
+  lds_fft_lines_t<-1>(lds, N + 1, nl, plan, tw, threadIdx.x, blockDim.x);
   for (int i = threadIdx.x; i < 2 * N * cols; i += blockDim.x) {
     const int ky = i / cols, cc = i - ky * cols;
     const int kp = ky - N;                    // centred k' in [-N, N)
     const int par = kp & 1;                   // odd -> twisted line
     const int m = (kp - par) / 2;             // [-N/2, N/2)
     const int b = m < 0 ? m + N : m;
-    Bg[((long)z * 2 * N + ky) * (2 * N) + k0 + cc] = lds[(2 * cc + par) * N + b];
+    Bg[((long)z * 2 * N + ky) * (2 * N) + k0 + cc] = lds[(2 * cc + par) * (N + 1) + b];
   }
 }
 
@@ -222,24 +222,28 @@ __global__ __launch_bounds__(256) void k_gather_tile(float2* __restrict__ G,
 }
 
 // centred inverse 1D DFT of each (angle, slice) line of gathered samples,
-// fftshift1c signs folded into index rotations, detector crop
+// fftshift1c signs folded into index rotations, detector crop; zq slices of
+// one angle per workgroup (lines padded to N+1 in LDS)
 __global__ __launch_bounds__(256) void k_line_idft(float2* __restrict__ g, const float2* __restrict__ G,
                             const float2* __restrict__ tw, LinePlan plan, float oscale, int n,
-                            int ntheta, int pz) {
+                            int ntheta, int pz, int zq) {
   extern __shared__ float2 lds[];
-  const int N = plan.N, th = blockIdx.x, z = blockIdx.y;
-  const float2* Gl = G + (long)z * ntheta * N + (long)th * N;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) {
-    int a = i - N / 2;  // kappa -> kappa mod N
+  const int N = plan.N, th = blockIdx.x, z0 = blockIdx.y * zq;
+  const int nzl = min(zq, pz - z0);
+  for (int i = threadIdx.x; i < zq * N; i += blockDim.x) {
+    const int zz = i / N, k = i - zz * N;
+    int a = k - N / 2;  // kappa -> kappa mod N
     if (a < 0) a += N;
-    lds[a] = Gl[i];
+    lds[zz * (N + 1) + a] =
+        zz < nzl ? G[(long)(z0 + zz) * ntheta * N + (long)th * N + k] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  lds_fft_lines_t<1>(lds, N, 1, plan, tw, threadIdx.x, blockDim.x);
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+  lds_fft_lines_t<1>(lds, N + 1, zq, plan, tw, threadIdx.x, blockDim.x);
+  for (int i = threadIdx.x; i < nzl * n; i += blockDim.x) {
+    const int zz = i / n, j = i - zz * n;
     int b = j - n / 2;
     if (b < 0) b += N;
-    g[((long)th * pz + z) * n + j] = cscale(lds[b], oscale);
+    g[((long)th * pz + z0 + zz) * n + j] = cscale(lds[zz * (N + 1) + b], oscale);
   }
 }
 
@@ -263,17 +267,17 @@ __global__ __launch_bounds__(256) void k_adj_line(float2* __restrict__ Q, const 
     }
     int a = sidx + N / 2;
     if (a >= N) a -= N;
-    lds[zz * N + a] = v;
+    lds[zz * (N + 1) + a] = v;
   }
   __syncthreads();
-  lds_fft_lines_t<-1>(lds, N, zq, plan, tw, threadIdx.x, blockDim.x);
+  lds_fft_lines_t<-1>(lds, N + 1, zq, plan, tw, threadIdx.x, blockDim.x);
   for (int t = threadIdx.x; t < zq * N; t += blockDim.x) {
     const int i = t / zq, zz = t - i * zq;
     if (zz >= nzl) continue;
     int b = i - N / 2;  // kappa
     if (b < 0) b += N;
     const float w = filt ? oscale * filt[i] : oscale;
-    Q[((long)th * N + i) * pz + z0 + zz] = cscale(lds[zz * N + b], w);
+    Q[((long)th * N + i) * pz + z0 + zz] = cscale(lds[zz * (N + 1) + b], w);
   }
 }
 
@@ -394,18 +398,18 @@ __global__ __launch_bounds__(256) void k_col_adj(float2* __restrict__ A, const f
     const int par = kp & 1;
     const int m = (kp - par) / 2;
     const int b = m < 0 ? m + N : m;
-    lds[(2 * cc + par) * N + b] = Bg[((long)z * 2 * N + ky) * (2 * N) + k0 + cc];
+    lds[(2 * cc + par) * (N + 1) + b] = Bg[((long)z * 2 * N + ky) * (2 * N) + k0 + cc];
   }
   __syncthreads();
-  lds_fft_lines_t<1>(lds, N, nl, plan, tw, threadIdx.x, blockDim.x);
+  lds_fft_lines_t<1>(lds, N + 1, nl, plan, tw, threadIdx.x, blockDim.x);
   const int o = N / 6;
   for (int i = threadIdx.x; i < n * cols; i += blockDim.x) {
     const int qi = i / cols, cc = i - qi * cols;
     const int q = qi + o;             // row index, centred q - N/2
     int a = q - N / 2;
     if (a < 0) a += N;
-    const float2 e = lds[(2 * cc) * N + a];
-    const float2 od = lds[(2 * cc + 1) * N + a];
+    const float2 e = lds[(2 * cc) * (N + 1) + a];
+    const float2 od = lds[(2 * cc + 1) * (N + 1) + a];
     A[((long)z * n + qi) * (2 * N) + k0 + cc] = cadd(e, cmulc(od, twist[q]));
   }
 }
@@ -428,10 +432,10 @@ __global__ __launch_bounds__(256) void k_row_adj(float2* __restrict__ f, const f
     const int par = kp & 1;
     const int m = (kp - par) / 2;
     const int b = m < 0 ? m + N : m;
-    lds[(2 * r + par) * N + b] = val;
+    lds[(2 * r + par) * (N + 1) + b] = val;
   }
   __syncthreads();
-  lds_fft_lines_t<1>(lds, N, nl, plan, tw, threadIdx.x, blockDim.x);
+  lds_fft_lines_t<1>(lds, N + 1, nl, plan, tw, threadIdx.x, blockDim.x);
   for (int i = threadIdx.x; i < n * rows; i += blockDim.x) {
     const int r = i / n, pi = i - r * n;
     const int qi = r0 + r;
@@ -439,8 +443,8 @@ __global__ __launch_bounds__(256) void k_row_adj(float2* __restrict__ f, const f
     const int q = qi + o, p = pi + o;
     int a = p - N / 2;
     if (a < 0) a += N;
-    const float2 e = lds[(2 * r) * N + a];
-    const float2 od = lds[(2 * r + 1) * N + a];
+    const float2 e = lds[(2 * r) * (N + 1) + a];
+    const float2 od = lds[(2 * r + 1) * (N + 1) + a];
     float2 v = cadd(e, cmulc(od, twist[p]));
     const int2 mr = mrange[q];
     const float w = (p >= mr.x && p < mr.y) ? dphi[q] * dphi[p] * scale : 0.f;
@@ -453,7 +457,7 @@ int lines_fit(const LinePlan& p, int nthr, int maxlines, size_t lds_cap) {
   int best = 1;
   for (int l = 1; l <= maxlines; ++l) {
     if (!lds_fft_fits(p, l, nthr)) break;
-    if ((size_t)l * p.N * sizeof(float2) > lds_cap) break;
+    if ((size_t)l * (p.N + 1) * sizeof(float2) > lds_cap) break;  // padded lines
     best = l;
   }
   return best;
@@ -465,7 +469,7 @@ int run_fwd(pt_tomo* h, float2* g, const float2* f, float scale, hipStream_t st)
   {
     const int rows = h->rows_per_wg;
     dim3 grid((N + rows - 1) / rows, h->pz);
-    size_t lds = sizeof(float2) * 2 * rows * N;
+    size_t lds = sizeof(float2) * 2 * rows * (N + 1);
     ProfScope prof("tomo_row_fwd", st);
     hipLaunchKernelGGL(k_row_fwd, grid, dim3(nthr), lds, st, h->A, f, h->tw, h->twist, h->dphi,
                        h->mrange, h->plan, h->n, rows);
@@ -474,7 +478,7 @@ int run_fwd(pt_tomo* h, float2* g, const float2* f, float scale, hipStream_t st)
   {
     const int cols = h->cols_per_wg;
     dim3 grid(2 * N / cols, h->pz);
-    size_t lds = sizeof(float2) * 2 * cols * N;
+    size_t lds = sizeof(float2) * 2 * cols * (N + 1);
     ProfScope prof("tomo_col_fwd", st);
     hipLaunchKernelGGL(k_col_fwd, grid, dim3(nthr), lds, st, h->Bg, h->A, h->tw, h->twist, h->plan,
                        cols);
@@ -493,12 +497,11 @@ int run_fwd(pt_tomo* h, float2* g, const float2* f, float scale, hipStream_t st)
   {
     const double sgn = ((N / 2) % 2) ? -1.0 : 1.0;
     const float oscale = (float)(sgn * kPiRef / (double)h->mu / (double)N * (double)scale);
-    dim3 grid(h->ntheta, h->pz);
-    int thr = std::min(256, ((N + 63) / 64) * 64);
-    if (!lds_fft_fits(h->plan, 1, thr)) thr = 256;
+    const int zq = h->adj_lines;    // detector lines per workgroup (fits the LDS FFT)
+    dim3 grid(h->ntheta, (h->pz + zq - 1) / zq);
     ProfScope prof("tomo_line_idft", st);
-    hipLaunchKernelGGL(k_line_idft, grid, dim3(thr), sizeof(float2) * N, st, g, h->Q, h->tw,
-                       h->plan, oscale, h->n, h->ntheta, h->pz);
+    hipLaunchKernelGGL(k_line_idft, grid, dim3(256), sizeof(float2) * zq * (N + 1), st, g, h->Q,
+                       h->tw, h->plan, oscale, h->n, h->ntheta, h->pz, zq);
     PT_LAUNCH_CHECK();
   }
   return PT_OK;
@@ -513,7 +516,7 @@ int run_adj(pt_tomo* h, float2* f, const float2* g, float scale, hipStream_t st)
     const int zq = h->adj_lines;
     dim3 grid(h->ntheta, (h->pz + zq - 1) / zq);
     ProfScope prof("tomo_adj_line", st);
-    hipLaunchKernelGGL(k_adj_line, grid, dim3(256), sizeof(float2) * zq * N, st, h->Q, g, h->tw,
+    hipLaunchKernelGGL(k_adj_line, grid, dim3(256), sizeof(float2) * zq * (N + 1), st, h->Q, g, h->tw,
                        h->filt, h->plan, oscale, h->n, h->ntheta, h->pz, zq);
     PT_LAUNCH_CHECK();
   }
@@ -537,7 +540,7 @@ int run_adj(pt_tomo* h, float2* f, const float2* g, float scale, hipStream_t st)
   {
     const int cols = h->cols_per_wg;
     dim3 grid(2 * N / cols, h->pz);
-    size_t lds = sizeof(float2) * 2 * cols * N;
+    size_t lds = sizeof(float2) * 2 * cols * (N + 1);
     ProfScope prof("tomo_col_adj", st);
     hipLaunchKernelGGL(k_col_adj, grid, dim3(nthr), lds, st, h->A, h->Bg, h->tw, h->twist, h->plan,
                        h->n, cols);
@@ -546,7 +549,7 @@ int run_adj(pt_tomo* h, float2* f, const float2* g, float scale, hipStream_t st)
   {
     const int rows = h->rows_per_wg;
     dim3 grid((h->n + rows - 1) / rows, h->pz);
-    size_t lds = sizeof(float2) * 2 * rows * N;
+    size_t lds = sizeof(float2) * 2 * rows * (N + 1);
     ProfScope prof("tomo_row_adj", st);
     hipLaunchKernelGGL(k_row_adj, grid, dim3(nthr), lds, st, f, h->A, h->tw, h->twist, h->dphi,
                        h->mrange, h->plan, h->n, rows, scale);
@@ -866,7 +869,7 @@ extern "C" int pt_tomo_set_tuning(pt_tomo* h, int rows, int cols, int adj_lines)
   const size_t cap = 64 * 1024;
   auto ok = [&](int lines) {
     return lines >= 1 && lds_fft_fits(h->plan, lines, 256) &&
-           (size_t)lines * N * sizeof(float2) <= cap;
+           (size_t)lines * (N + 1) * sizeof(float2) <= cap;
   };
   PT_REQUIRE(ok(2 * rows), PT_ERR_SIZE, "pt_tomo_set_tuning: rows per workgroup do not fit");
   PT_REQUIRE(ok(2 * cols) && (2 * N) % cols == 0, PT_ERR_SIZE,
