@@ -143,8 +143,9 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
-        print(json.dumps({"error": "this build targets one GPU; multi-GPU runs are not built "
-                                   "(see DESIGN.md for how the path would shard)",
+        print(json.dumps({"error": "one GPU only: the ADMM path would shard over angle "
+                                   "chunks (ptychography) and slice blocks (tomography), "
+                                   "which is not built",
                           "n_gpus": max(args.gpus, world)}))
         sys.exit(2)
 

@@ -67,7 +67,7 @@ def tune(op, nz, n, ntheta, cache=CACHE, log=None):
     # one parameter at a time: rows (row pass, forward), cols (column
     # passes, both), detector lines (line passes, both)
     both = lambda: (fwd(), adj())
-    for idx, cands, fn in ((0, [1, 2, 3, 4, 6, 8], fwd), (1, [1, 2, 4, 8], both),
+    for idx, cands, fn in ((0, [1, 2, 3, 4, 6, 8], fwd), (1, [2, 4, 8, 16, 32], both),
                            (2, [1, 2, 4, 8], both)):
         res = {}
         for v in cands:

@@ -225,26 +225,33 @@ __host__ __device__ __forceinline__ void reg_fft(float2 (&v)[L]) {
 }
 
 // ---------------------------------------------------------------- LDS mixed-radix FFT
-// Run-time plan for one line length N = prod(radix[i]).  Twiddle table
-// tw[m] = exp(-2 pi i m / N) lives in global memory (FP64-rounded).
+// Division by a run-time invariant d (Granlund & Montgomery, "Division by
+// invariant integers using multiplication", 1994, round-up variant):
+// x / d == (umulhi(x, m) + x) >> s for 0 <= x < 2^31, with
+// s = ceil(log2 d) and m = floor(2^32 (2^s - d) / d) + 1.
+struct DivC {
+  unsigned m, s;
+};
+inline DivC make_divc(unsigned d) {
+  unsigned s = 0;
+  while ((1u << s) < d) ++s;
+  const unsigned long long m = ((1ull << 32) * ((1ull << s) - d)) / d + 1;
+  return DivC{(unsigned)m, s};
+}
+__device__ __forceinline__ int fdiv(int x, DivC c) {
+  return (int)((__umulhi((unsigned)x, c.m) + (unsigned)x) >> c.s);
+}
+
+// Run-time plan for one line length N = prod(radix[i]), with the division
+// constants of each pass: by its butterfly count N / radix[i] and by its
+// input stride Ns = radix[0] * ... * radix[i-1].  Twiddle table
+// tw[m] = exp(-2 pi i m / N) (FP64-rounded) in global memory or staged in LDS.
 struct LinePlan {
   int N;
   int npass;
   int radix[12];
+  DivC dnb[12], dns[12];
 };
-
-#define PT_LDS_FFT_MAXV 16
-
-// host: true if every pass of `p` fits the per-thread register budget
-inline bool lds_fft_fits(const LinePlan& p, int nlines, int nthr) {
-  for (int i = 0; i < p.npass; ++i) {
-    int R = p.radix[i];
-    long nb = (long)nlines * (p.N / R);
-    long per = (nb + nthr - 1) / nthr;
-    if (per * R > PT_LDS_FFT_MAXV) return false;
-  }
-  return true;
-}
 
 template <int SIGN>
 __device__ __forceinline__ void dft3(float2* a) {
@@ -284,81 +291,64 @@ __device__ __forceinline__ void dft_small(float2* a) {
   else dft_ct<R, SIGN>(a);
 }
 
-// One in-place Stockham pass of radix R over `nlines` lines; each thread
-// keeps at most PT_LDS_FFT_MAXV values (compile-time bound, so they stay in
-// VGPRs).  Twiddle W_{Ns R}^{i k} = tw[i k N/(Ns R)] (conjugated for SIGN=+1).
+// One out-of-place Stockham pass of radix R over `nlines` lines (src and
+// dst at line * stride): each thread takes butterflies b = t, t + nthr, ...
+// one at a time (R values live), so the pass needs no register budget and
+// one barrier.  Twiddle W_{Ns R}^{i k} = tw[i k N/(Ns R)] (conjugated for
+// SIGN=+1); tw[0] = 1 exactly, so the k = 0 products are exact.
 template <int R, int SIGN>
-__device__ __forceinline__ void lds_pass(float2* buf, int stride, int nlines, int N, int Ns,
-                                         const float2* __restrict__ tw, int t, int nthr) {
-  constexpr int MAXB = PT_LDS_FFT_MAXV / R;
+__device__ __forceinline__ void pp_pass(const float2* __restrict__ src, float2* __restrict__ dst,
+                                        int stride, int nlines, int N, int Ns, DivC dnb, DivC dns,
+                                        const float2* __restrict__ tw, int t, int nthr) {
   const int nb = N / R;
   const int total = nb * nlines;
-  const int step = N / (Ns * R);
-  float2 v[MAXB][R];
+  const int step = nb / Ns;
+  for (int b = t; b < total; b += nthr) {
+    const int line = fdiv(b, dnb), j = b - line * nb;
+    const int jq = fdiv(j, dns), k = j - jq * Ns;
+    const float2* s = src + line * stride + j;
+    float2 v[R];
 #pragma unroll
-  for (int bi = 0; bi < MAXB; ++bi) {
-    const int b = t + bi * nthr;
-    if (b < total) {
-      const int line = b / nb, j = b - line * nb;
-      const int k = j % Ns;
-      const float2* src = buf + line * stride;
+    for (int i = 0; i < R; ++i) v[i] = s[i * nb];
 #pragma unroll
-      for (int i = 0; i < R; ++i) {
-        float2 x = src[j + i * nb];
-        if (i) {
-          const int m = i * k * step;
-          if (m) {
-            float2 w = tw[m];
-            if (SIGN > 0) w.y = -w.y;
-            x = cmul(x, w);
-          }
-        }
-        v[bi][i] = x;
-      }
-      dft_small<R, SIGN>(v[bi]);
+    for (int i = 1; i < R; ++i) {
+      float2 w = tw[i * k * step];
+      if (SIGN > 0) w.y = -w.y;
+      v[i] = cmul(v[i], w);
     }
-  }
-  __syncthreads();
+    dft_small<R, SIGN>(v);
+    float2* d = dst + line * stride + jq * Ns * R + k;
 #pragma unroll
-  for (int bi = 0; bi < MAXB; ++bi) {
-    const int b = t + bi * nthr;
-    if (b < total) {
-      const int line = b / nb, j = b - line * nb;
-      const int k = j % Ns;
-      float2* dst = buf + line * stride;
-      const int idxD = (j / Ns) * Ns * R + k;
-#pragma unroll
-      for (int i = 0; i < R; ++i) dst[idxD + i * Ns] = v[bi][i];
-    }
+    for (int i = 0; i < R; ++i) d[i * Ns] = v[i];
   }
-  __syncthreads();
 }
 
+// FFT of `nlines` lines of length plan.N held in LDS at a + line*stride,
+// ping-ponging with b (same shape), by `nthr` threads (index t) that all
+// call it together; begins with the caller's barrier already passed and
+// ends with one.  Returns the buffer that holds the result (a or b).
+// sign -1 forward, +1 inverse (unnormalised).
 template <int SIGN>
-__device__ inline void lds_fft_lines_t(float2* buf, int stride, int nlines, const LinePlan& plan,
-                                       const float2* __restrict__ tw, int t, int nthr) {
+__device__ inline float2* lds_fft_pp(float2* a, float2* b, int stride, int nlines,
+                                     const LinePlan& plan, const float2* __restrict__ tw, int t,
+                                     int nthr) {
   int Ns = 1;
   for (int p = 0; p < plan.npass; ++p) {
-    const int R = plan.radix[p];
-    switch (R) {
-      case 2: lds_pass<2, SIGN>(buf, stride, nlines, plan.N, Ns, tw, t, nthr); break;
-      case 3: lds_pass<3, SIGN>(buf, stride, nlines, plan.N, Ns, tw, t, nthr); break;
-      case 4: lds_pass<4, SIGN>(buf, stride, nlines, plan.N, Ns, tw, t, nthr); break;
-      case 5: lds_pass<5, SIGN>(buf, stride, nlines, plan.N, Ns, tw, t, nthr); break;
-      default: lds_pass<8, SIGN>(buf, stride, nlines, plan.N, Ns, tw, t, nthr); break;
+    const DivC dnb = plan.dnb[p], dns = plan.dns[p];
+    switch (plan.radix[p]) {
+      case 2: pp_pass<2, SIGN>(a, b, stride, nlines, plan.N, Ns, dnb, dns, tw, t, nthr); break;
+      case 3: pp_pass<3, SIGN>(a, b, stride, nlines, plan.N, Ns, dnb, dns, tw, t, nthr); break;
+      case 4: pp_pass<4, SIGN>(a, b, stride, nlines, plan.N, Ns, dnb, dns, tw, t, nthr); break;
+      case 5: pp_pass<5, SIGN>(a, b, stride, nlines, plan.N, Ns, dnb, dns, tw, t, nthr); break;
+      default: pp_pass<8, SIGN>(a, b, stride, nlines, plan.N, Ns, dnb, dns, tw, t, nthr); break;
     }
-    Ns *= R;
+    __syncthreads();
+    Ns *= plan.radix[p];
+    float2* c = a;
+    a = b;
+    b = c;
   }
-}
-
-// In-place FFT of `nlines` lines of length plan.N stored in LDS at
-// buf + line*stride, executed by `nthr` threads (thread index t) that all call
-// it together; contains __syncthreads().  sign = -1 forward, +1 inverse
-// (unnormalised).  The caller guarantees lds_fft_fits(plan, nlines, nthr).
-__device__ inline void lds_fft_lines(float2* buf, int stride, int nlines, const LinePlan& plan,
-                                     const float2* __restrict__ tw, int sign, int t, int nthr) {
-  if (sign < 0) lds_fft_lines_t<-1>(buf, stride, nlines, plan, tw, t, nthr);
-  else lds_fft_lines_t<1>(buf, stride, nlines, plan, tw, t, nthr);
+  return a;
 }
 
 // ---------------------------------------------------------------- reductions

@@ -93,6 +93,9 @@ __global__ __launch_bounds__(256) void k_row_fwd(float2* __restrict__ A, const f
   const int N = plan.N, z = blockIdx.y, q0 = blockIdx.x * rows;
   const int o = N / 6;
   const int nl = 2 * rows;  // line 2r: plain, 2r+1: twisted
+  float2* lb = lds + nl * (N + 1);  // ping-pong partner
+  float2* twl = lb + nl * (N + 1);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
   for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
     const int line = i / N, c = i - line * N;
     const int r = line >> 1, q = q0 + r;
@@ -111,43 +114,81 @@ __global__ __launch_bounds__(256) void k_row_fwd(float2* __restrict__ A, const f
     lds[line * (N + 1) + a] = val;
   }
   __syncthreads();
-  lds_fft_lines_t<-1>(lds, N + 1, nl, plan, tw, threadIdx.x, blockDim.x);
+  const float2* res = lds_fft_pp<-1>(lds, lb, N + 1, nl, plan, twl, threadIdx.x, blockDim.x);
   for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
     const int line = i / N, b = i - line * N;
     const int r = line >> 1, q = q0 + r;
     if (q >= N) continue;
     const int m = b < N / 2 ? b : b - N;  // centred frequency
     const int kx = 2 * m + N + (line & 1);
-    A[((long)z * N + q) * (2 * N) + kx] = lds[line * (N + 1) + b];
+    A[((long)z * N + q) * (2 * N) + kx] = res[line * (N + 1) + b];
   }
 }
 
 // ------------------------------------------------------------------ forward column pass
-// For columns kx of slice z: y[q] = A[z][q][kx] -> centred 2N-point DFT along
-// q -> B[z][ky][kx].
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  xcd_pair renumbers linear workgroup ids so that logical tiles 2i and
+// 2i+1 run on one XCD back to back (they share the cache lines of the rows
+// they read): w -> (w & ~15) | (w & 7) << 1 | (w >> 3) & 1, a bijection on
+// every aligned group of 16 ids; ids of a ragged last group stay.
+__device__ __forceinline__ int xcd_pair(int w, int total) {
+  if (w >= (total & ~15)) return w;
+  return (w & ~15) | ((w & 7) << 1) | ((w >> 3) & 1);
+}
+
+// C = 2^lc columns kx of one slice per workgroup: y[q] = A[z][q][kx] ->
+// centred 2N-point DFT along q -> B[z][ky][kx], as the plain N-point DFT
+// (even ky - N) and the twisted one (odd ky - N) run one after the other in
+// the same C padded LDS lines (ping-pong pair).  The N x C tile stays in
+// registers between the two; the twiddle table is staged in LDS.
+constexpr int kColVpt = 12;  // tile values per thread: N * C <= 256 * kColVpt
 __global__ __launch_bounds__(256) void k_col_fwd(float2* __restrict__ Bg, const float2* __restrict__ A,
                           const float2* __restrict__ tw, const float2* __restrict__ twist,
-                          LinePlan plan, int cols) {
+                          LinePlan plan, int lc) {
   extern __shared__ float2 lds[];
-  const int N = plan.N, z = blockIdx.y, k0 = blockIdx.x * cols;
-  const int nl = 2 * cols;
-  for (int i = threadIdx.x; i < N * cols; i += blockDim.x) {
-    const int q = i / cols, cc = i - q * cols;
-    const float2 val = A[((long)z * N + q) * (2 * N) + k0 + cc];
-    int a = q + N / 2;
-    if (a >= N) a -= N;
-    lds[(2 * cc) * (N + 1) + a] = val;
-    lds[(2 * cc + 1) * (N + 1) + a] = cmul(val, twist[q]);
+  const int N = plan.N, twoN = 2 * N, S = N + 1, C = 1 << lc;
+  const int gx = (twoN + C - 1) >> lc;
+  const int L = xcd_pair(blockIdx.x + gx * blockIdx.y, gx * gridDim.y);
+  const int z = L / gx, k0 = (L - z * gx) << lc;
+  float2* la = lds;
+  float2* lb = la + C * S;
+  float2* twl = lb + C * S;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
+  const int nv = (N * C + 255) >> 8;
+  const float2* Az = A + (long)z * N * twoN + k0;
+  float2 reg[kColVpt];
+#pragma unroll
+  for (int v = 0; v < kColVpt; ++v) {
+    if (v < nv) {
+      const int i = threadIdx.x + v * 256;
+      const int q = i >> lc, cc = i & (C - 1);
+      reg[v] = (q < N && k0 + cc < twoN) ? Az[(long)q * twoN + cc] : make_float2(0.f, 0.f);
+    }
   }
-  __syncthreads();
-  lds_fft_lines_t<-1>(lds, N + 1, nl, plan, tw, threadIdx.x, blockDim.x);
-  for (int i = threadIdx.x; i < 2 * N * cols; i += blockDim.x) {
-    const int ky = i / cols, cc = i - ky * cols;
-    const int kp = ky - N;                    // centred k' in [-N, N)
-    const int par = kp & 1;                   // odd -> twisted line
-    const int m = (kp - par) / 2;             // [-N/2, N/2)
-    const int b = m < 0 ? m + N : m;
-    Bg[((long)z * 2 * N + ky) * (2 * N) + k0 + cc] = lds[(2 * cc + par) * (N + 1) + b];
+  float2* Bz = Bg + (long)z * twoN * twoN + k0;
+  for (int par = 0; par < 2; ++par) {
+#pragma unroll
+    for (int v = 0; v < kColVpt; ++v) {
+      if (v < nv) {
+        const int i = threadIdx.x + v * 256;
+        const int q = i >> lc, cc = i & (C - 1);
+        if (q < N) {
+          int a = q + N / 2;  // centred -> rotated
+          if (a >= N) a -= N;
+          la[cc * S + a] = par ? cmul(reg[v], twist[q]) : reg[v];
+        }
+      }
+    }
+    __syncthreads();
+    const float2* res = lds_fft_pp<-1>(la, lb, S, C, plan, twl, threadIdx.x, blockDim.x);
+    for (int i = threadIdx.x; i < N * C; i += blockDim.x) {
+      const int r = i >> lc, cc = i & (C - 1);
+      if (k0 + cc >= twoN) continue;
+      const int m = r - N / 2;  // ky - N = 2m + par
+      const int b = m < 0 ? m + N : m;
+      Bz[(long)(2 * m + par + N) * twoN + cc] = res[cc * S + b];
+    }
+    __syncthreads();
   }
 }
 
@@ -230,6 +271,9 @@ __global__ __launch_bounds__(256) void k_line_idft(float2* __restrict__ g, const
   extern __shared__ float2 lds[];
   const int N = plan.N, th = blockIdx.x, z0 = blockIdx.y * zq;
   const int nzl = min(zq, pz - z0);
+  float2* lb = lds + zq * (N + 1);
+  float2* twl = lb + zq * (N + 1);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
   for (int i = threadIdx.x; i < zq * N; i += blockDim.x) {
     const int zz = i / N, k = i - zz * N;
     int a = k - N / 2;  // kappa -> kappa mod N
@@ -238,12 +282,12 @@ __global__ __launch_bounds__(256) void k_line_idft(float2* __restrict__ g, const
         zz < nzl ? G[(long)(z0 + zz) * ntheta * N + (long)th * N + k] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  lds_fft_lines_t<1>(lds, N + 1, zq, plan, tw, threadIdx.x, blockDim.x);
+  const float2* res = lds_fft_pp<1>(lds, lb, N + 1, zq, plan, twl, threadIdx.x, blockDim.x);
   for (int i = threadIdx.x; i < nzl * n; i += blockDim.x) {
     const int zz = i / n, j = i - zz * n;
     int b = j - n / 2;
     if (b < 0) b += N;
-    g[((long)th * pz + z0 + zz) * n + j] = cscale(lds[zz * (N + 1) + b], oscale);
+    g[((long)th * pz + z0 + zz) * n + j] = cscale(res[zz * (N + 1) + b], oscale);
   }
 }
 
@@ -258,6 +302,9 @@ __global__ __launch_bounds__(256) void k_adj_line(float2* __restrict__ Q, const 
   const int N = plan.N, th = blockIdx.x, z0 = blockIdx.y * zq;
   const int nzl = min(zq, pz - z0);
   const int o = N / 6;
+  float2* lb = lds + zq * (N + 1);
+  float2* twl = lb + zq * (N + 1);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
   for (int i = threadIdx.x; i < zq * N; i += blockDim.x) {
     const int zz = i / N, sidx = i - zz * N;
     float2 v = make_float2(0.f, 0.f);
@@ -270,14 +317,14 @@ __global__ __launch_bounds__(256) void k_adj_line(float2* __restrict__ Q, const 
     lds[zz * (N + 1) + a] = v;
   }
   __syncthreads();
-  lds_fft_lines_t<-1>(lds, N + 1, zq, plan, tw, threadIdx.x, blockDim.x);
+  const float2* res = lds_fft_pp<-1>(lds, lb, N + 1, zq, plan, twl, threadIdx.x, blockDim.x);
   for (int t = threadIdx.x; t < zq * N; t += blockDim.x) {
     const int i = t / zq, zz = t - i * zq;
     if (zz >= nzl) continue;
     int b = i - N / 2;  // kappa
     if (b < 0) b += N;
     const float w = filt ? oscale * filt[i] : oscale;
-    Q[((long)th * N + i) * pz + z0 + zz] = cscale(lds[zz * (N + 1) + b], w);
+    Q[((long)th * N + i) * pz + z0 + zz] = cscale(res[zz * (N + 1) + b], w);
   }
 }
 
@@ -384,33 +431,59 @@ __global__ __launch_bounds__(64) void k_scatter_fix(float2* __restrict__ Bg,
 }
 
 // ------------------------------------------------------------------ adjoint column pass
-// For columns kx: centred inverse 2N-point DFT along ky evaluated at the n
-// object rows q in [o, o+n): even/odd split into two N-point inverse DFTs.
+// C = 2^lc columns kx per workgroup: centred inverse 2N-point DFT along ky
+// evaluated at the n object rows q in [o, o+n), as the even and the odd
+// N-point inverse DFTs run one after the other in the same C LDS lines
+// (ping-pong pair); the even half at the output rows waits in registers.
 __global__ __launch_bounds__(256) void k_col_adj(float2* __restrict__ A, const float2* __restrict__ Bg,
                           const float2* __restrict__ tw, const float2* __restrict__ twist,
-                          LinePlan plan, int n, int cols) {
+                          LinePlan plan, int n, int lc) {
   extern __shared__ float2 lds[];
-  const int N = plan.N, z = blockIdx.y, k0 = blockIdx.x * cols;
-  const int nl = 2 * cols;
-  for (int i = threadIdx.x; i < 2 * N * cols; i += blockDim.x) {
-    const int ky = i / cols, cc = i - ky * cols;
-    const int kp = ky - N;
-    const int par = kp & 1;
-    const int m = (kp - par) / 2;
-    const int b = m < 0 ? m + N : m;
-    lds[(2 * cc + par) * (N + 1) + b] = Bg[((long)z * 2 * N + ky) * (2 * N) + k0 + cc];
+  const int N = plan.N, twoN = 2 * N, S = N + 1, o = N / 6, C = 1 << lc;
+  const int gx = (twoN + C - 1) >> lc;
+  const int L = xcd_pair(blockIdx.x + gx * blockIdx.y, gx * gridDim.y);
+  const int z = L / gx, k0 = (L - z * gx) << lc;
+  float2* la = lds;
+  float2* lb = la + C * S;
+  float2* twl = lb + C * S;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
+  const int nv = (n * C + 255) >> 8;  // n * C <= 256 * kColVpt since n = 2N/3
+  const float2* Bz = Bg + (long)z * twoN * twoN + k0;
+  float2* Az = A + (long)z * n * twoN + k0;
+  // lines of parity par of the C columns -> LDS, inverse FFT
+  auto phase = [&](int par) -> const float2* {
+    for (int i = threadIdx.x; i < N * C; i += blockDim.x) {
+      const int r = i >> lc, cc = i & (C - 1);
+      const int m = r - N / 2;  // ky - N = 2m + par
+      const int b = m < 0 ? m + N : m;
+      la[cc * S + b] = (k0 + cc < twoN) ? Bz[(long)(2 * m + par + N) * twoN + cc]
+                                        : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    return lds_fft_pp<1>(la, lb, S, C, plan, twl, threadIdx.x, blockDim.x);
+  };
+  float2 ev[kColVpt];
+  const float2* res = phase(0);
+#pragma unroll
+  for (int v = 0; v < kColVpt; ++v) {
+    const int i = threadIdx.x + v * 256;
+    const int qi = i >> lc, cc = i & (C - 1);
+    int a = qi + o - N / 2;
+    if (a < 0) a += N;
+    if (v < nv && qi < n) ev[v] = res[cc * S + a];
   }
   __syncthreads();
-  lds_fft_lines_t<1>(lds, N + 1, nl, plan, tw, threadIdx.x, blockDim.x);
-  const int o = N / 6;
-  for (int i = threadIdx.x; i < n * cols; i += blockDim.x) {
-    const int qi = i / cols, cc = i - qi * cols;
-    const int q = qi + o;             // row index, centred q - N/2
-    int a = q - N / 2;
-    if (a < 0) a += N;
-    const float2 e = lds[(2 * cc) * (N + 1) + a];
-    const float2 od = lds[(2 * cc + 1) * (N + 1) + a];
-    A[((long)z * n + qi) * (2 * N) + k0 + cc] = cadd(e, cmulc(od, twist[q]));
+  res = phase(1);
+#pragma unroll
+  for (int v = 0; v < kColVpt; ++v) {
+    const int i = threadIdx.x + v * 256;
+    const int qi = i >> lc, cc = i & (C - 1);
+    if (v < nv && qi < n && k0 + cc < twoN) {
+      const int q = qi + o;  // row index, centred q - N/2
+      int a = q - N / 2;
+      if (a < 0) a += N;
+      Az[(long)qi * twoN + cc] = cadd(ev[v], cmulc(res[cc * S + a], twist[q]));
+    }
   }
 }
 
@@ -423,6 +496,9 @@ __global__ __launch_bounds__(256) void k_row_adj(float2* __restrict__ f, const f
   const int N = plan.N, z = blockIdx.y, r0 = blockIdx.x * rows;
   const int o = N / 6;
   const int nl = 2 * rows;
+  float2* lb = lds + nl * (N + 1);
+  float2* twl = lb + nl * (N + 1);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
   for (int i = threadIdx.x; i < 2 * N * rows; i += blockDim.x) {
     const int r = i / (2 * N), kx = i - r * 2 * N;
     const int qi = r0 + r;
@@ -435,7 +511,7 @@ __global__ __launch_bounds__(256) void k_row_adj(float2* __restrict__ f, const f
     lds[(2 * r + par) * (N + 1) + b] = val;
   }
   __syncthreads();
-  lds_fft_lines_t<1>(lds, N + 1, nl, plan, tw, threadIdx.x, blockDim.x);
+  const float2* res = lds_fft_pp<1>(lds, lb, N + 1, nl, plan, twl, threadIdx.x, blockDim.x);
   for (int i = threadIdx.x; i < n * rows; i += blockDim.x) {
     const int r = i / n, pi = i - r * n;
     const int qi = r0 + r;
@@ -443,8 +519,8 @@ __global__ __launch_bounds__(256) void k_row_adj(float2* __restrict__ f, const f
     const int q = qi + o, p = pi + o;
     int a = p - N / 2;
     if (a < 0) a += N;
-    const float2 e = lds[(2 * r) * (N + 1) + a];
-    const float2 od = lds[(2 * r + 1) * (N + 1) + a];
+    const float2 e = res[(2 * r) * (N + 1) + a];
+    const float2 od = res[(2 * r + 1) * (N + 1) + a];
     float2 v = cadd(e, cmulc(od, twist[p]));
     const int2 mr = mrange[q];
     const float w = (p >= mr.x && p < mr.y) ? dphi[q] * dphi[p] * scale : 0.f;
@@ -453,14 +529,32 @@ __global__ __launch_bounds__(256) void k_row_adj(float2* __restrict__ f, const f
 }
 
 // ------------------------------------------------------------------ host helpers
-int lines_fit(const LinePlan& p, int nthr, int maxlines, size_t lds_cap) {
+// LDS of a line kernel: nl padded lines, their ping-pong partners and the
+// twiddle table
+size_t line_lds_bytes(int N, int nl) { return sizeof(float2) * (2 * (size_t)nl * (N + 1) + N); }
+// LDS of a column kernel: C padded lines, their partners and the twiddle table
+size_t col_lds_bytes(int N, int C) { return sizeof(float2) * (2 * (size_t)C * (N + 1) + N); }
+constexpr size_t kLdsCap = 64 * 1024;
+
+int lines_fit(const LinePlan& p, int maxlines, size_t lds_cap) {
   int best = 1;
   for (int l = 1; l <= maxlines; ++l) {
-    if (!lds_fft_fits(p, l, nthr)) break;
-    if ((size_t)l * (p.N + 1) * sizeof(float2) > lds_cap) break;  // padded lines
+    if (line_lds_bytes(p.N, l) > lds_cap) break;
     best = l;
   }
   return best;
+}
+
+// C columns per workgroup of the column passes: a power of two whose tile
+// fits the per-thread registers and the LDS
+bool col_fits(const LinePlan& p, int C) {
+  if (C < 1 || (C & (C - 1))) return false;
+  return (long)p.N * C <= 256L * kColVpt && col_lds_bytes(p.N, C) <= kLdsCap;
+}
+int ilog2(int c) {
+  int l = 0;
+  while ((1 << l) < c) ++l;
+  return l;
 }
 
 int run_fwd(pt_tomo* h, float2* g, const float2* f, float scale, hipStream_t st) {
@@ -469,7 +563,7 @@ int run_fwd(pt_tomo* h, float2* g, const float2* f, float scale, hipStream_t st)
   {
     const int rows = h->rows_per_wg;
     dim3 grid((N + rows - 1) / rows, h->pz);
-    size_t lds = sizeof(float2) * 2 * rows * (N + 1);
+    size_t lds = line_lds_bytes(N, 2 * rows);
     ProfScope prof("tomo_row_fwd", st);
     hipLaunchKernelGGL(k_row_fwd, grid, dim3(nthr), lds, st, h->A, f, h->tw, h->twist, h->dphi,
                        h->mrange, h->plan, h->n, rows);
@@ -477,11 +571,10 @@ int run_fwd(pt_tomo* h, float2* g, const float2* f, float scale, hipStream_t st)
   }
   {
     const int cols = h->cols_per_wg;
-    dim3 grid(2 * N / cols, h->pz);
-    size_t lds = sizeof(float2) * 2 * cols * (N + 1);
+    dim3 grid((2 * N + cols - 1) / cols, h->pz);
     ProfScope prof("tomo_col_fwd", st);
-    hipLaunchKernelGGL(k_col_fwd, grid, dim3(nthr), lds, st, h->Bg, h->A, h->tw, h->twist, h->plan,
-                       cols);
+    hipLaunchKernelGGL(k_col_fwd, grid, dim3(nthr), col_lds_bytes(N, cols), st, h->Bg, h->A, h->tw,
+                       h->twist, h->plan, ilog2(cols));
     PT_LAUNCH_CHECK();
   }
   {
@@ -500,7 +593,7 @@ int run_fwd(pt_tomo* h, float2* g, const float2* f, float scale, hipStream_t st)
     const int zq = h->adj_lines;    // detector lines per workgroup (fits the LDS FFT)
     dim3 grid(h->ntheta, (h->pz + zq - 1) / zq);
     ProfScope prof("tomo_line_idft", st);
-    hipLaunchKernelGGL(k_line_idft, grid, dim3(256), sizeof(float2) * zq * (N + 1), st, g, h->Q,
+    hipLaunchKernelGGL(k_line_idft, grid, dim3(256), line_lds_bytes(N, zq), st, g, h->Q,
                        h->tw, h->plan, oscale, h->n, h->ntheta, h->pz, zq);
     PT_LAUNCH_CHECK();
   }
@@ -516,7 +609,7 @@ int run_adj(pt_tomo* h, float2* f, const float2* g, float scale, hipStream_t st)
     const int zq = h->adj_lines;
     dim3 grid(h->ntheta, (h->pz + zq - 1) / zq);
     ProfScope prof("tomo_adj_line", st);
-    hipLaunchKernelGGL(k_adj_line, grid, dim3(256), sizeof(float2) * zq * (N + 1), st, h->Q, g, h->tw,
+    hipLaunchKernelGGL(k_adj_line, grid, dim3(256), line_lds_bytes(N, zq), st, h->Q, g, h->tw,
                        h->filt, h->plan, oscale, h->n, h->ntheta, h->pz, zq);
     PT_LAUNCH_CHECK();
   }
@@ -539,17 +632,16 @@ int run_adj(pt_tomo* h, float2* f, const float2* g, float scale, hipStream_t st)
   }
   {
     const int cols = h->cols_per_wg;
-    dim3 grid(2 * N / cols, h->pz);
-    size_t lds = sizeof(float2) * 2 * cols * (N + 1);
+    dim3 grid((2 * N + cols - 1) / cols, h->pz);
     ProfScope prof("tomo_col_adj", st);
-    hipLaunchKernelGGL(k_col_adj, grid, dim3(nthr), lds, st, h->A, h->Bg, h->tw, h->twist, h->plan,
-                       h->n, cols);
+    hipLaunchKernelGGL(k_col_adj, grid, dim3(nthr), col_lds_bytes(N, cols), st, h->A, h->Bg, h->tw,
+                       h->twist, h->plan, h->n, ilog2(cols));
     PT_LAUNCH_CHECK();
   }
   {
     const int rows = h->rows_per_wg;
     dim3 grid((h->n + rows - 1) / rows, h->pz);
-    size_t lds = sizeof(float2) * 2 * rows * (N + 1);
+    size_t lds = line_lds_bytes(N, 2 * rows);
     ProfScope prof("tomo_row_adj", st);
     hipLaunchKernelGGL(k_row_adj, grid, dim3(nthr), lds, st, f, h->A, h->tw, h->twist, h->dphi,
                        h->mrange, h->plan, h->n, rows, scale);
@@ -581,6 +673,12 @@ bool make_line_plan(int N, LinePlan* p) {
     p->radix[p->npass++] = r;
     rem /= r;
   }
+  int Ns = 1;
+  for (int i = 0; i < p->npass; ++i) {
+    p->dnb[i] = make_divc((unsigned)(N / p->radix[i]));
+    p->dns[i] = make_divc((unsigned)Ns);
+    Ns *= p->radix[i];
+  }
   return true;
 }
 
@@ -597,7 +695,8 @@ extern "C" int pt_tomo_create(int ntheta, int pnz, int n, pt_tomo** out) {
   LinePlan plan;
   PT_REQUIRE(make_line_plan(N, &plan), PT_ERR_SIZE,
              "pt_tomo_create: 3n/2 must factor into radices 2, 3, 4, 5, 8");
-  PT_REQUIRE(lds_fft_fits(plan, 1, 256), PT_ERR_SIZE, "pt_tomo_create: n too large");
+  PT_REQUIRE(line_lds_bytes(N, 2) <= kLdsCap && col_fits(plan, 1), PT_ERR_SIZE,
+             "pt_tomo_create: n too large");
   pt_tomo* h = new pt_tomo();
   h->ntheta = ntheta; h->pz = pnz; h->n = n; h->N = N; h->plan = plan;
   // radonusfft.cu:10-12
@@ -611,14 +710,15 @@ extern "C" int pt_tomo_create(int ntheta, int pnz, int n, pt_tomo** out) {
     delete h;
     return PT_ERR_SIZE;
   }
-  // LDS lines per workgroup for the row/column passes (<= 48 KB)
+  // LDS lines per workgroup of the row and detector-line passes (<= 48 KB);
+  // columns per workgroup of the column passes (up to 16: whole cache lines)
   const size_t cap = 48 * 1024;
-  int lines = lines_fit(plan, 256, 16, cap);
+  int lines = lines_fit(plan, 16, cap);
   h->rows_per_wg = std::max(1, lines / 2);
-  int cols = 1;
-  while (cols * 2 <= lines / 2 && (2 * N) % (cols * 2) == 0) cols *= 2;
+  int cols = 16;
+  while (cols > 1 && !col_fits(plan, cols)) cols /= 2;
   h->cols_per_wg = cols;
-  h->adj_lines = lines_fit(plan, 256, kZQ, cap);
+  h->adj_lines = lines_fit(plan, kZQ, cap);
   // host tables
   std::vector<float2> tw(N), twist(N);
   std::vector<float> dphi(N);
@@ -866,14 +966,12 @@ extern "C" int pt_tomo_get_tuning(pt_tomo* h, int* rows, int* cols, int* adj_lin
 extern "C" int pt_tomo_set_tuning(pt_tomo* h, int rows, int cols, int adj_lines) {
   PT_REQUIRE(h, PT_ERR_ARG, "pt_tomo_set_tuning: null handle");
   const int N = h->N;
-  const size_t cap = 64 * 1024;
   auto ok = [&](int lines) {
-    return lines >= 1 && lds_fft_fits(h->plan, lines, 256) &&
-           (size_t)lines * (N + 1) * sizeof(float2) <= cap;
+    return lines >= 1 && line_lds_bytes(N, lines) <= kLdsCap;
   };
   PT_REQUIRE(ok(2 * rows), PT_ERR_SIZE, "pt_tomo_set_tuning: rows per workgroup do not fit");
-  PT_REQUIRE(ok(2 * cols) && (2 * N) % cols == 0, PT_ERR_SIZE,
-             "pt_tomo_set_tuning: columns per workgroup do not fit or do not divide 2N");
+  PT_REQUIRE(col_fits(h->plan, cols), PT_ERR_SIZE,
+             "pt_tomo_set_tuning: columns per workgroup must be a power of two whose tile fits");
   PT_REQUIRE(ok(adj_lines) && adj_lines <= kZQ, PT_ERR_SIZE,
              "pt_tomo_set_tuning: adjoint lines per workgroup do not fit");
   h->rows_per_wg = rows;

@@ -51,7 +51,7 @@ class Workload:
                     seed=self.seed)
 
 
-# BASELINE configurations (the first is the CPU-runnable case, the second the
+# Benchmark configurations (the first is the CPU-runnable case, the second the
 # production single-GPU case, the last the large one)
 CONFIGS = {
     "plain64": Workload("plain ADMM 64^3, 96 angles, 16x16 probe, step 4", 64, 64, 96, 20,

@@ -4,8 +4,9 @@ The U-Net (``unet.UNet3D``) maps the (delta, beta) volume after a few ADMM
 iterations to the volume after many; its output replaces u in the short
 run's state (h and e are recomputed from it: h = exptomo(R u) e^{i pshift},
 e = J u) and ADMM continues from there.  Plain ADMM stays the fallback and
-the judge of the error: ``compare`` reports the time to reach the plain
-method's converged Lagrangian within a relative tolerance, for both.
+the measure of the error: ``compare`` times both runs to the plain method's
+converged Lagrangian (within a relative tolerance) and, since the phantom is
+known, to the plain method's final object error.
 
 Training pairs are made by the solver itself on seeded ellipsoid phantoms
 (``pairs``): input = u after ``n_short`` iterations, target = u after

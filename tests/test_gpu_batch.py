@@ -103,7 +103,7 @@ def test_autotune_shapes_bit_identical(tmp_path):
         op.fwd(a1.data_ptr(), u.data_ptr())
         op.adj(b1.data_ptr(), g.data_ptr())
         assert torch.equal(a0, a1) and torch.equal(b0, b1), s
-    assert not autotune._set(op, (1, 5, 1))       # 5 does not divide 2N = 192
+    assert not autotune._set(op, (1, 5, 1))       # columns: a power of two
     # cached: a second tune reads the file
     shape2, _ = autotune.tune(op, nz, n, ntheta, cache=str(tmp_path / "t.json"))
     assert shape2 == shape

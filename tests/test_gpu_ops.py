@@ -32,7 +32,7 @@ def make_scan(rng, pt, S, nz, n, P):
 
 @pytest.mark.parametrize("D,P,nz,n,S,pt", [(16, 16, 40, 48, 30, 3), (16, 12, 33, 41, 25, 2),
                                            (8, 5, 20, 24, 17, 2), (32, 32, 70, 64, 9, 2),
-                                           (64, 16, 80, 80, 6, 1)])
+                                           (64, 16, 80, 80, 6, 1), (16, 16, 64, 256, 400, 2)])
 def test_ptycho_fwd_adj(D, P, nz, n, S, pt):
     from ptychotomo_amd.ptychofft import ptychofft
     print(f"ptycho D={D} P={P} nz={nz} n={n} S={S} pt={pt}", flush=True)
@@ -66,7 +66,8 @@ def test_ptycho_fwd_adj(D, P, nz, n, S, pt):
     assert ad < 1e-5, ad
 
 
-@pytest.mark.parametrize("n,pz,ntheta", [(16, 2, 12), (32, 3, 20), (64, 2, 24), (96, 1, 8)])
+@pytest.mark.parametrize("n,pz,ntheta", [(16, 2, 12), (32, 3, 20), (64, 2, 24), (96, 1, 8),
+                                         (256, 2, 16), (512, 1, 6)])
 def test_tomo_fwd_adj(n, pz, ntheta):
     from ptychotomo_amd.radonusfft import radonusfft
     print(f"tomo n={n} pz={pz} ntheta={ntheta}", flush=True)
@@ -89,6 +90,27 @@ def test_tomo_fwd_adj(n, pz, ntheta):
     print(f"fwd rel {ef:.3e} adj rel {ea:.3e}")
     assert ef < 1e-5, ef
     assert ea < 1e-5, ea
+
+
+@pytest.mark.parametrize("n", [256, 384])
+def test_tomo_large_vs_direct_nudft(n):
+    """Production-size USFFT forward against the FP64 direct nonuniform DFT
+    on four angles of one slice (the USFFT targets eps = 1e-6)."""
+    from ptychotomo_amd.radonusfft import radonusfft
+    print(f"tomo direct NUDFT n={n}", flush=True)
+    rng = np.random.default_rng(n + 1)
+    theta = rng.uniform(0, np.pi, 4).astype(np.float32)
+    u = crandn(rng, (1, n, n))
+    op = radonusfft(4, 1, n)
+    th = dev(theta)
+    op.setobj(th.data_ptr())
+    out = torch.zeros((4, 1, n), dtype=torch.complex64, device="cuda")
+    ud = dev(u)
+    op.fwd(out.data_ptr(), ud.data_ptr())
+    ref = ousfft.fwd_exact(u, theta)
+    err = rel(out.cpu().numpy(), ref)
+    print(f"n={n}: USFFT vs FP64 direct NUDFT rel err {err:.2e}")
+    assert err < 1e-5, err
 
 
 def test_tomo_adjoint_zero_border():
