@@ -57,6 +57,35 @@ def kernel_bytes(tag, slv):
     return None
 
 
+def minf_work(history, nchunk, piter):
+    """Chunk evaluations done by the ptychography line-search kernel over
+    `history` (one ADMM iteration per record).  A sweep evaluates every chunk
+    whose search is still open (finished chunks exit at once), so the number
+    of sweeps a chunk took follows from its accepted step: the search starts
+    at 2 (then at the previous step), sweep s tries halvings k = 4s .. 4s+3,
+    and a step reaching 1e-12 ends at 0.  Lagrangian evaluations read every
+    chunk once (psi and data only)."""
+    import math
+    from ptychotomo_amd.solver import _PGAM, _PSWEEPS
+    per = _PGAM - 1
+    sweeps, lagr = 0, 0
+    for rec in history:
+        if rec.get("lagr"):
+            lagr += 1
+        for chunk_g in rec["ptycho_gammas"]:
+            g0 = 2.0
+            for g in chunk_g[:piter]:
+                if g0 <= 1e-12:
+                    k = 0
+                elif g > 0:
+                    k = int(round(math.log2(g0 / g)))
+                else:
+                    k = math.ceil(math.log2(g0 / 1e-12))
+                sweeps += min(k // per + 1, _PSWEEPS)
+                g0 = g
+    return sweeps, lagr
+
+
 TAGS = ["tomo_gather", "tomo_scatter", "tomo_fold", "tomo_row_fwd", "tomo_col_fwd",
         "tomo_adj_line", "tomo_col_adj", "tomo_row_adj", "ptycho_grad_tile", "ptycho_fold",
         "ptycho_minf"]
@@ -207,6 +236,19 @@ def main():
     avg_ms = tot.value / max(cnt.value, 1)
     nbytes = kernel_bytes(dom, slv)
     achieved = nbytes / (avg_ms * 1e-3) / 1e9 if nbytes else None
+    work = None
+    if dom == "ptycho_minf":
+        # its launches are line-search sweeps; chunks whose search has ended
+        # exit at once, so count the chunk evaluations actually done
+        nchunk = slv.ntheta // slv.ptheta
+        ev, nl = minf_work(slv.history, nchunk, w.piter)
+        vol_t = 8 * slv.ntheta * slv.nz * slv.n
+        dat = 4 * slv.ntheta * slv.nscan * slv.ndety * slv.ndety
+        done = ev * (2 * vol_t + dat) / nchunk + nl * (vol_t + dat)
+        achieved = done / (tot.value * 1e-3) / 1e9
+        work = {"chunk_evaluations": ev, "lagrangian_passes": nl, "nchunk": nchunk,
+                "full_launch_equivalents": (ev + nl * nchunk) / nchunk,
+                "bytes_done": done}
     traffic = None
     tf = os.path.join(HERE, "profiles", f"{dom}_{args.workload}_traffic.txt")
     if os.path.exists(tf):
@@ -232,7 +274,7 @@ def main():
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic, "algorithmic_bytes": nbytes, "avg_launch_ms": avg_ms,
-                     "launches": cnt.value},
+                     "launches": cnt.value, "kernel_ms_total": tot.value, "work": work},
         "breakdown_ms_per_iter": {k: round(v["ms_per_iter"], 3) for k, v in breakdown.items()},
         "datagen_s": round(t_gen, 2),
     }

@@ -393,6 +393,143 @@ void k_minf(double* __restrict__ partial,
   }
 }
 
+// Line-search sums of k_minf (lagr = 0), organised by the overlap-add
+// tiles: a workgroup of kMinfWaves waves takes one tile, stages the psi and
+// d pixels of its EB x EB extended region once in LDS as (psi, d) pairs
+// (coalesced rows; the tile's patches then read their frames from LDS
+// instead of each loading D overlapping rows from memory) and evaluates the
+// tile's patches G at a time, wave w taking groups w, w + kMinfWaves, ...
+// Per-patch arithmetic is that of k_minf; one partial per (chunk, tile,
+// wave) in partial[((c*tpc + tile)*kMinfWaves + w)*ngam + j].
+constexpr int kMinfWaves = 2;
+constexpr int kMinfStage = 16;  // staged pixels per thread: EB*EB <= 64*kMinfWaves*kMinfStage
+template <int D, int MODEL>
+__global__ __launch_bounds__(WAVE * kMinfWaves) __attribute__((amdgpu_waves_per_eu(D <= 16 ? 3 : 1)))
+void k_minf_tile(
+    double* __restrict__ partial, const float2* __restrict__ f, const float2* __restrict__ dir,
+    const float* __restrict__ data, const float2* __restrict__ prb, const int2* __restrict__ pos,
+    const int* __restrict__ tile_start, const int* __restrict__ tile_patch,
+    const double* __restrict__ gammas, int ngam, int chunk, int nty, int ntx, int B, int EB, int nz,
+    int n, int P, int ox, int oy, float cs, float cp, float cd, const double* __restrict__ skip) {
+  constexpr int G = WAVE / D;
+  // LDS: EB*EB (psi, d) pairs, the probe (P*P), then the transpose buffers
+  extern __shared__ float4 lds_pd[];
+  const int c = blockIdx.y;
+  if (skip && skip[2 * c + 1] != 0.0) return;
+  const int wave = threadIdx.x / WAVE, lane = lane_id(), slot = lane / D, r = lane % D;
+  float2* prl = reinterpret_cast<float2*>(lds_pd + EB * EB);
+  float* T = reinterpret_cast<float*>(prl + ((P * P + 1) & ~1)) + wave * G * D * (D + 1);
+  for (int i = threadIdx.x; i < P * P; i += blockDim.x) prl[i] = prb[i];
+  __shared__ float gsh[PT_MAX_GAM];
+  const int tpa = nty * ntx, tpc = chunk * tpa;
+  const long tile = (long)c * tpc + blockIdx.x;
+  const int t = (int)(tile / tpa);
+  const int rem = (int)(tile - (long)t * tpa);
+  const int y0 = (rem / ntx) * B, x0 = (rem % ntx) * B;
+  const int beg = tile_start[tile], end = tile_start[tile + 1];
+  if (threadIdx.x < PT_MAX_GAM)
+    gsh[threadIdx.x] = threadIdx.x < ngam ? (float)gammas[(long)c * ngam + threadIdx.x] : 0.f;
+  if (end > beg) {
+    // all loads of the region first (one memory round trip), then the stores
+    const float2* fz = f + (long)t * nz * n;
+    const float2* dz = dir + (long)t * nz * n;
+    float2 va[kMinfStage], vb[kMinfStage];
+#pragma unroll
+    for (int u = 0; u < kMinfStage; ++u) {
+      const int i = threadIdx.x + u * WAVE * kMinfWaves;
+      const int yy = i / EB, xx = i - yy * EB;
+      const int y = y0 + yy, x = x0 + xx;
+      const bool in = i < EB * EB && y < nz && x < n;
+      va[u] = in ? fz[(long)y * n + x] : make_float2(0.f, 0.f);
+      vb[u] = in ? dz[(long)y * n + x] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < kMinfStage; ++u) {
+      const int i = threadIdx.x + u * WAVE * kMinfWaves;
+      if (i < EB * EB) lds_pd[i] = make_float4(va[u].x, va[u].y, vb[u].x, vb[u].y);
+    }
+  }
+  __syncthreads();
+  double acc[PT_MAX_GAM];
+#pragma unroll
+  for (int j = 0; j < PT_MAX_GAM; ++j) acc[j] = 0.0;
+  const int ix = r - ox;
+  for (int i0 = beg + wave * G; i0 < end; i0 += kMinfWaves * G) {
+    const int ii = i0 + slot;
+    const bool active = ii < end;
+    const long p = active ? tile_patch[ii] : 0;
+    const int2 ps = active ? pos[p] : make_int2(0, 0);
+    float2 a[D], b[D];
+    float d[D];
+    {
+      const float* dp = data + p * D * D + r;
+#pragma unroll
+      for (int k = 0; k < D; ++k) d[k] = active ? dp[k * D] : 0.f;
+    }
+    const bool rowin = active && ix >= 0 && ix < P;
+    const float4* src = lds_pd + (ps.y - oy - y0) * EB + ps.x + ix - x0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const int iy = k - oy;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (rowin && iy >= 0 && iy < P) v = src[k * EB];
+      a[k] = make_float2(v.x, v.y);
+      b[k] = make_float2(v.z, v.w);
+    }
+    apply_probe<D>(a, prl, r, P, ox, oy, cs);
+    frame_fwd_w<D>(a, T, slot, r);
+    apply_probe<D>(b, prl, r, P, ox, oy, cs);
+    frame_fwd_w<D>(b, T, slot, r);
+    if (!active) continue;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      d[k] = d[k] * cd;
+      a[k] = cscale(a[k], cp);
+      b[k] = cscale(b[k], cp);
+      if (MODEL == PT_MODEL_GAUSSIAN) d[k] = sqrtf(d[k]);
+    }
+#pragma unroll 1
+    for (int j = 0; j < PT_MAX_GAM; ++j) {
+      if (j < ngam && gsh[j] != 0.f) acc[0] += (double)delta_part<D, MODEL>(a, b, d, gsh[j]);
+      const double t0 = acc[0];
+#pragma unroll
+      for (int q = 0; q < PT_MAX_GAM - 1; ++q) acc[q] = acc[q + 1];
+      acc[PT_MAX_GAM - 1] = t0;
+    }
+  }
+  const long nbt = (long)tpc * kMinfWaves;
+  double* out = partial + (long)c * ngam * nbt + (long)blockIdx.x * kMinfWaves + wave;
+#pragma unroll
+  for (int j = 0; j < PT_MAX_GAM; ++j) {
+    if (j >= ngam) break;
+    const double v = wave_sum(acc[j]);
+    if (lane == 0) out[j * nbt] = v;
+  }
+}
+
+// Sums of the k_minf_tile partials (layout [chunk][gamma][block]): one
+// 256-thread workgroup per (chunk, gamma), thread-strided sums in block
+// order, then the fixed-shape workgroup tree (deterministic).
+__global__ __launch_bounds__(256) void k_reduce_partials_t(double* __restrict__ out,
+                                                           const double* __restrict__ partial,
+                                                           long nblk, int ngam,
+                                                           const double* __restrict__ skip) {
+  __shared__ double sm[16];
+  const int c = blockIdx.x, j = blockIdx.y;
+  if (skip && skip[2 * c + 1] != 0.0) return;
+  const double* src = partial + ((long)c * ngam + j) * nblk;
+  double v[1] = {0.0};
+  for (long b = threadIdx.x; b < nblk; b += blockDim.x) v[0] += src[b];
+  block_sum<1>(v, sm);
+  if (threadIdx.x == 0) out[c * ngam + j] = v[0];
+}
+
+// LDS bytes of a k_minf_tile workgroup
+inline size_t minf_tile_lds_bytes(int D, int EB, int P) {
+  return sizeof(float4) * (size_t)EB * EB + sizeof(float2) * (size_t)((P * P + 1) & ~1) +
+         sizeof(float) * (size_t)kMinfWaves * (64 / D) * D * (D + 1);
+}
+
 __global__ __launch_bounds__(64) void k_reduce_partials(double* __restrict__ out,
                                                          const double* __restrict__ partial,
                                                          int nblk, int ngam,
@@ -656,6 +793,38 @@ int run_minf(pt_ptycho* h, double* sums, const float2* f, const float2* dir, con
              const double* skip, hipStream_t st) {
   constexpr int G = WAVE / D;
   const int nchunk = h->pt / chunk;
+  // line-search sums over the overlap-add tiles (frames read from LDS)
+  const size_t tl = minf_tile_lds_bytes(D, h->EB, h->P);
+  if constexpr (D <= 32) if (!lagr && dir && tl <= 64 * 1024 &&
+                            h->EB * h->EB <= WAVE * kMinfWaves * kMinfStage) {
+    const int tpc = chunk * h->nty * h->ntx;
+    const int nbt = tpc * kMinfWaves;
+    size_t need = (size_t)nchunk * nbt * ngam;
+    if (need > h->partial_cap) {
+      if (h->partial) PT_HIP(hipFree(h->partial));
+      h->partial = nullptr;
+      PT_HIP(hipMalloc(&h->partial, need * sizeof(double)));
+      h->partial_cap = need;
+    }
+    {
+      ProfScope prof("ptycho_minf", st);
+      if (model == PT_MODEL_POISSON)
+        hipLaunchKernelGGL((k_minf_tile<D, PT_MODEL_POISSON>), dim3(tpc, nchunk),
+                           dim3(WAVE * kMinfWaves), tl, st, h->partial, f, dir, data, h->prb, h->pos,
+                           h->tile_start, h->tile_patch, gammas, ngam, chunk, h->nty, h->ntx, h->B,
+                           h->EB, h->nz, h->n, h->P, h->ox, h->oy, h->c, cp, cd, skip);
+      else
+        hipLaunchKernelGGL((k_minf_tile<D, PT_MODEL_GAUSSIAN>), dim3(tpc, nchunk),
+                           dim3(WAVE * kMinfWaves), tl, st, h->partial, f, dir, data, h->prb, h->pos,
+                           h->tile_start, h->tile_patch, gammas, ngam, chunk, h->nty, h->ntx, h->B,
+                           h->EB, h->nz, h->n, h->P, h->ox, h->oy, h->c, cp, cd, skip);
+    }
+    PT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_reduce_partials_t, dim3(nchunk, ngam), dim3(256), 0, st, sums, h->partial,
+                       (long)nbt, ngam, skip);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+  }
   long np = (long)chunk * h->S;
   int nb = launch_blocks(np, 4 * G, std::max(1, 20480 / nchunk));
   size_t need = (size_t)nchunk * nb * ngam;

@@ -197,6 +197,7 @@ __global__ __launch_bounds__(256) void k_col_fwd(float2* __restrict__ Bg, const 
 // an M-point halo (periodic in 2N) is staged in LDS as [y][x][z-pair]; each
 // lane interpolates one polar sample of the bin with separable Gaussian
 // weights (kernels.cuh:97-126).  Output G[z][sample].
+constexpr int kGatherStage = 9;  // staged points per thread: (kTile + 14)^2 <= 2304
 __global__ __launch_bounds__(256) void k_gather_tile(float2* __restrict__ G,
                                                      const float2* __restrict__ Bg,
                                                      const float2* __restrict__ spos,
@@ -214,13 +215,25 @@ __global__ __launch_bounds__(256) void k_gather_tile(float2* __restrict__ G,
   const int ty0 = (tile / ntile1) * kTile - N, tx0 = (tile % ntile1) * kTile - N;
   const float2* B0 = Bg + (long)z0 * twoN * twoN;
   const float2* B1 = two ? B0 + (long)twoN * twoN : B0;
-  for (int i = threadIdx.x; i < E * E; i += blockDim.x) {
-    const int y = i / E, x = i - y * E;
-    const int gy = wrap2n(N + ty0 - M + y, twoN), gx = wrap2n(N + tx0 - M + x, twoN);
-    const long off = (long)gy * twoN + gx;
-    const float2 a = B0[off];
-    const float2 b = two ? B1[off] : make_float2(0.f, 0.f);
-    lds4[i] = make_float4(a.x, a.y, b.x, b.y);
+  // every load of the staged tile first (one memory round trip), then the
+  // LDS stores: E*E <= 256*kGatherStage
+  {
+    float2 va[kGatherStage], vb[kGatherStage];
+#pragma unroll
+    for (int u = 0; u < kGatherStage; ++u) {
+      const int i = threadIdx.x + u * 256;
+      const int y = i / E, x = i - y * E;
+      const int gy = wrap2n(N + ty0 - M + y, twoN), gx = wrap2n(N + tx0 - M + x, twoN);
+      const long off = (long)gy * twoN + gx;
+      const bool in = i < E * E;
+      va[u] = in ? B0[off] : make_float2(0.f, 0.f);
+      vb[u] = in && two ? B1[off] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < kGatherStage; ++u) {
+      const int i = threadIdx.x + u * 256;
+      if (i < E * E) lds4[i] = make_float4(va[u].x, va[u].y, vb[u].x, vb[u].y);
+    }
   }
   __syncthreads();
   const int2 rg = wi_range[wi];
@@ -366,30 +379,53 @@ __global__ __launch_bounds__(64) void k_scatter_own(float2* __restrict__ Bg,
   float2 acc[kZS];
 #pragma unroll
   for (int j = 0; j < kZS; ++j) acc[j] = make_float2(0.f, 0.f);
+  // software pipeline over the chunks: the entry index of chunk i+2 and the
+  // sample data of chunk i+1 are loaded while chunk i is accumulated, so a
+  // chunk's staging waits on no memory round trip
+  const int el = lane & (kSE - 1);
+  auto fetch_idx = [&](int c0) -> int { return c0 + el < rg.y ? ent_s[c0 + el] : -1; };
+  float2 pn = make_float2(0.f, 0.f);
+  int2 fln = make_int2(0, 0);
+  int on = 0;
+  float2 qn[kZS];
+  auto fetch_data = [&](int c0, int s) {
+    if (s < 0) return;
+    if (lane < kSE) {
+      pn = spos[s];
+      fln = sfloor[s];
+      on = ent_o[c0 + el];
+    } else {
+      const float2* qs = Q + (long)s * pz + z0;
+#pragma unroll
+      for (int j = 0; j < kZS; ++j) qn[j] = (z0 + j < pz) ? qs[j] : make_float2(0.f, 0.f);
+    }
+  };
+  int sc = fetch_idx(rg.x);
+  fetch_data(rg.x, sc);
+  int sn = fetch_idx(rg.x + kSE);
   for (int c0 = rg.x; c0 < rg.y; c0 += kSE) {
     const int ne = min(kSE, rg.y - c0);
     __syncthreads();
-    const int el = lane & (kSE - 1);
-    if (el < ne) {
-      const int s = ent_s[c0 + el];
+    if (sc >= 0) {
       if (lane < kSE) {
-        const float2 p = spos[s];
-        const int2 fl = sfloor[s];
 #pragma unroll
         for (int k = 0; k < 15; ++k) {
-          const float dx = (float)(fl.x - M + k) * inv2N - p.x;
-          const float dy = (float)(fl.y - M + k) * inv2N - p.y;
+          const float dx = (float)(fln.x - M + k) * inv2N - pn.x;
+          const float dy = (float)(fln.y - M + k) * inv2N - pn.y;
           wxs[el][k] = (k <= 2 * M) ? __expf(-beta * dx * dx) : 0.f;
           wys[el][k] = (k <= 2 * M) ? __expf(-beta * dy * dy) : 0.f;
         }
-        eos[el] = ent_o[c0 + el];
+        eos[el] = on;
       } else {
-        const float2* qs = Q + (long)s * pz + z0;
 #pragma unroll
-        for (int j = 0; j < kZS; ++j) gs[el][j] = (z0 + j < pz) ? qs[j] : make_float2(0.f, 0.f);
+        for (int j = 0; j < kZS; ++j) gs[el][j] = qn[j];
       }
     }
     __syncthreads();
+    // prefetch: data of the next chunk, index of the one after
+    sc = sn;
+    fetch_data(c0 + kSE, sc);
+    sn = fetch_idx(c0 + 2 * kSE);
     for (int e = 0; e < ne; ++e) {
       const int o = eos[e];
       const int k0 = lx - ((o & 255) - 64);

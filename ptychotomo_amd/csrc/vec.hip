@@ -51,7 +51,8 @@ ProfScope::~ProfScope() {
 namespace {
 
 constexpr int RB = 256;          // threads per reduction block
-constexpr int RMAXBLK = 512;     // blocks per segment cap
+constexpr int RMAXBLK = 8192;    // blocks per segment cap
+constexpr int RMAXTOT = 16384;   // blocks per reduction cap (all segments)
 
 // scratch for partial sums, one per stream (grown on demand): reductions
 // enqueued on different streams may run concurrently
@@ -97,28 +98,19 @@ __global__ __launch_bounds__(RB) void k_seg_reduce(double* __restrict__ partial,
   }
 }
 
-// second pass: one wave per (segment, accumulator): lane-strided sums in block
-// order, then a fixed-shape wave tree (deterministic)
-__global__ __launch_bounds__(64) void k_finish(double* __restrict__ out,
-                                               const double* __restrict__ partial, int nblk,
-                                               int nv) {
-  const int seg = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
-  double s = 0.0;
-  for (int b = lane; b < nblk; b += 64) s += partial[((long)seg * nblk + b) * nv + k];
-  s = wave_sum(s);
-  if (lane == 0) out[seg * nv + k] = s;
-}
-
-__global__ __launch_bounds__(64) void k_finish_n(double* __restrict__ out,
+// second pass: one workgroup per (segment, accumulator): thread-strided sums
+// in block order, then the fixed-shape workgroup tree (deterministic)
+__global__ __launch_bounds__(RB) void k_finish_n(double* __restrict__ out,
                                                  const double* __restrict__ partial, int nblk,
                                                  int nv, int nout,
                                                  const double* __restrict__ skip) {
-  const int seg = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
+  __shared__ double sm[16];
+  const int seg = blockIdx.x, k = blockIdx.y;
   if (skip && skip[2 * seg + 1] != 0.0) return;
-  double s = 0.0;
-  for (int b = lane; b < nblk; b += 64) s += partial[((long)seg * nblk + b) * nv + k];
-  s = wave_sum(s);
-  if (lane == 0) out[seg * nout + k] = s;
+  double s[1] = {0.0};
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s[0] += partial[((long)seg * nblk + b) * nv + k];
+  block_sum<1>(s, sm);
+  if (threadIdx.x == 0) out[seg * nout + k] = s[0];
 }
 
 template <int NV, typename F>
@@ -126,14 +118,15 @@ int seg_reduce(double* out, long len, int nseg, F f, hipStream_t st) {
   PT_REQUIRE(nseg > 0 && len % nseg == 0, PT_ERR_SIZE, "reduction: nseg must divide len");
   const long seglen = len / nseg;
   int nblk = (int)std::min<long>(RMAXBLK, std::max<long>(1, (seglen + RB * 4 - 1) / (RB * 4)));
-  nblk = std::max(1, std::min(nblk, std::max(1, 2048 / nseg)));
+  nblk = std::max(1, std::min(nblk, std::max(1, RMAXTOT / nseg)));
   double* part;
   int rc = partial_buf((size_t)nseg * nblk * NV, &part, st);
   if (rc) return rc;
   hipLaunchKernelGGL((k_seg_reduce<NV, F>), dim3(nblk, nseg), dim3(RB), 0, st, part, seglen, f,
                      (const double*)nullptr);
   PT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_finish, dim3(nseg, NV), dim3(64), 0, st, out, part, nblk, NV);
+  hipLaunchKernelGGL(k_finish_n, dim3(nseg, NV), dim3(RB), 0, st, out, part, nblk, NV, NV,
+                     (const double*)nullptr);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -147,14 +140,14 @@ int seg_reduce_n(double* out, long len, int nseg, int nout, F f, hipStream_t st,
   PT_REQUIRE(nseg > 0 && len % nseg == 0, PT_ERR_SIZE, "reduction: nseg must divide len");
   const long seglen = len / nseg;
   int nblk = (int)std::min<long>(RMAXBLK, std::max<long>(1, (seglen + RB * 4 - 1) / (RB * 4)));
-  nblk = std::max(1, std::min(nblk, std::max(1, 2048 / nseg)));
+  nblk = std::max(1, std::min(nblk, std::max(1, RMAXTOT / nseg)));
   double* part;
   int rc = partial_buf((size_t)nseg * nblk * NV, &part, st);
   if (rc) return rc;
   hipLaunchKernelGGL((k_seg_reduce<NV, F>), dim3(nblk, nseg), dim3(RB), 0, st, part, seglen, f,
                      skip);
   PT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_finish_n, dim3(nseg, nout), dim3(64), 0, st, out, part, nblk, NV, nout,
+  hipLaunchKernelGGL(k_finish_n, dim3(nseg, nout), dim3(RB), 0, st, out, part, nblk, NV, nout,
                      skip);
   PT_LAUNCH_CHECK();
   return PT_OK;
