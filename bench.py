@@ -86,6 +86,8 @@ def minf_work(history, nchunk, piter):
     return sweeps, lagr
 
 
+WARM_WEIGHTS = os.path.join(HERE, "ptychotomo_amd", "weights", "unet_warmstart_128.npz")
+
 TAGS = ["tomo_gather", "tomo_scatter", "tomo_fold", "tomo_row_fwd", "tomo_col_fwd",
         "tomo_adj_line", "tomo_col_adj", "tomo_row_adj", "ptycho_grad_tile", "ptycho_fold",
         "ptycho_minf"]
@@ -165,10 +167,14 @@ def main():
                     help="only run 5 device copies of MIB MiB (HBM counter calibration)")
     ap.add_argument("--tol-run", type=int, default=0,
                     help="also run N plain iterations for time-to-Lagrangian-tolerance")
-    ap.add_argument("--warm-weights", default=None,
-                    help="U-Net weights: time plain ADMM and the learned warm start to the "
-                         "plain method's converged Lagrangian (--tol-iters iterations)")
+    ap.add_argument("--warm-weights", default=WARM_WEIGHTS,
+                    help="U-Net weights: time plain ADMM and the learned warm start on the "
+                         "--tol-n^3 warm-start workload (--tol-iters iterations each)")
+    ap.add_argument("--tol-n", type=int, default=128,
+                    help="size of the plain-vs-warm comparison (plain ADMM does not converge "
+                         "on the 256^3 workload)")
     ap.add_argument("--tol-iters", type=int, default=100)
+    ap.add_argument("--no-warm", action="store_true", help="skip the plain-vs-warm comparison")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -247,15 +253,19 @@ def main():
         done = ev * (2 * vol_t + dat) / nchunk + nl * (vol_t + dat)
         achieved = done / (tot.value * 1e-3) / 1e9
         work = {"chunk_evaluations": ev, "lagrangian_passes": nl, "nchunk": nchunk,
-                "full_launch_equivalents": (ev + nl * nchunk) / nchunk,
-                "bytes_done": done}
+                "full_sweeps": ev / nchunk, "bytes_done": done,
+                "bytes_per_launch": done / max(cnt.value, 1)}
+        nbytes = done / max(cnt.value, 1)
     traffic = None
     tf = os.path.join(HERE, "profiles", f"{dom}_{args.workload}_traffic.txt")
     if os.path.exists(tf):
         with open(tf) as fh:
             for line in fh:
-                if line.startswith("hbm_bytes_per_launch:"):
+                if line.startswith("hbm_bytes_per_launch:") and dom != "ptycho_minf":
                     traffic = float(line.split(":", 1)[1])
+                if line.startswith("hbm_bytes_per_full_sweep:") and work:
+                    # per launch on the basis of this run's sweeps
+                    traffic = float(line.split(":", 1)[1]) * work["full_sweeps"] / max(cnt.value, 1)
 
     out = {
         "metric": "ADMM iters/s + time-to-Lagrangian-tol, 256^3, plain vs warm-start, 1 MI355X",
@@ -283,13 +293,14 @@ def main():
         out["lagr_last"] = lag[-1]
     if args.tol_run:
         out["time_to_tol"] = time_to_tol(slv, data, w, run, args.tol_run)
-    if args.warm_weights:
+    if args.warm_weights and not args.no_warm and os.path.exists(args.warm_weights):
         from ptychotomo_amd import warmstart
-        log(f"warm start: plain vs U-Net, {args.tol_iters} iterations each")
+        log(f"warm start: plain vs U-Net at {args.tol_n}^3, {args.tol_iters} iterations each")
         del slv, data
         torch.cuda.empty_cache()
         net = warmstart.load(args.warm_weights)
-        out["time_to_tol"] = warmstart.compare(net, w, n_ref=args.tol_iters,
+        wt = warmstart.workload(args.tol_n, seed=10_000 + args.tol_n, photons=1e3)
+        out["time_to_tol"] = warmstart.compare(net, wt, n_ref=args.tol_iters,
                                                n_max=args.tol_iters, tol=1e-3)
     if not args.no_cpu_baseline:
         log("cpu baseline")

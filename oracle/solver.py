@@ -273,13 +273,15 @@ class Solver:
 
     # solver.py:308-336
     def admm(self, data, h, e, psi, phi, lamd, mu, u, alpha, piter, titer, niter,
-             model, lagr_every=10, verbose=False):
+             model, lagr_every=10, verbose=False, m0=0):
+        """m0 (not in the reference): index of the first iteration; the first two
+        iterations of a run (m0 + m < 2) take piter + 32 ptychography CG steps."""
         rho = 0.5
         tau = 0.5
         self.history = []
         for m in range(niter):
             h0, e0 = h, e
-            psi = self.cg_ptycho_batch(data, psi, h, lamd, rho, piter + (m < 2) * 32, model)
+            psi = self.cg_ptycho_batch(data, psi, h, lamd, rho, piter + (m0 + m < 2) * 32, model)
             xi0, xi1, K, pshift = self.takexi(psi, phi, lamd, mu, rho, tau)
             u = self.cg_tomo(xi0, xi1, K, u, rho, tau, titer)
             phi = self.solve_reg(u, mu, tau, alpha)
