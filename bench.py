@@ -63,8 +63,8 @@ def minf_work(history, nchunk, piter):
     whose search is still open (finished chunks exit at once), so the number
     of sweeps a chunk took follows from its accepted step: the search starts
     at 2 (then at the previous step), sweep s tries halvings k = 4s .. 4s+3,
-    and a step reaching 1e-12 ends at 0.  Lagrangian evaluations read every
-    chunk once (psi and data only)."""
+    and a step reaching 1e-12 ends at 0.  Also returns the number of
+    Lagrangian evaluations (launched under their own tag)."""
     import math
     from ptychotomo_amd.solver import _PGAM, _PSWEEPS
     per = _PGAM - 1
@@ -74,7 +74,7 @@ def minf_work(history, nchunk, piter):
             lagr += 1
         for chunk_g in rec["ptycho_gammas"]:
             g0 = 2.0
-            for g in chunk_g[:piter]:
+            for g in chunk_g:
                 if g0 <= 1e-12:
                     k = 0
                 elif g > 0:
@@ -175,6 +175,9 @@ def main():
                          "on the 256^3 workload)")
     ap.add_argument("--tol-iters", type=int, default=100)
     ap.add_argument("--no-warm", action="store_true", help="skip the plain-vs-warm comparison")
+    ap.add_argument("--kernel", default=None,
+                    help="roofline kernel tag (default: the largest of the breakdown probe; "
+                         "ptycho_minf without the probe)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -225,8 +228,8 @@ def main():
         ms, cnt = prof_run(lib, tag, lambda: slv.admm_resume(data, niter=1, lagr_every=10, **run))
         breakdown[tag] = {"ms_per_iter": ms, "launches": cnt}
     slv.state = dict(saved)
-    dom = (max(breakdown, key=lambda k: breakdown[k]["ms_per_iter"]) if breakdown
-           else "tomo_scatter")
+    dom = args.kernel or (max(breakdown, key=lambda k: breakdown[k]["ms_per_iter"])
+                          if breakdown else "ptycho_minf")
     log(f"breakdown {breakdown}; timing {args.steps} iterations, roofline kernel {dom}")
 
     # timed region
@@ -250,7 +253,7 @@ def main():
         ev, nl = minf_work(slv.history, nchunk, w.piter)
         vol_t = 8 * slv.ntheta * slv.nz * slv.n
         dat = 4 * slv.ntheta * slv.nscan * slv.ndety * slv.ndety
-        done = ev * (2 * vol_t + dat) / nchunk + nl * (vol_t + dat)
+        done = ev * (2 * vol_t + dat) / nchunk
         achieved = done / (tot.value * 1e-3) / 1e9
         work = {"chunk_evaluations": ev, "lagrangian_passes": nl, "nchunk": nchunk,
                 "full_sweeps": ev / nchunk, "bytes_done": done,

@@ -403,7 +403,7 @@ void k_minf(double* __restrict__ partial,
 // wave) in partial[((c*tpc + tile)*kMinfWaves + w)*ngam + j].
 constexpr int kMinfWaves = 2;
 constexpr int kMinfStage = 16;  // staged pixels per thread: EB*EB <= 64*kMinfWaves*kMinfStage
-template <int D, int MODEL>
+template <int D, int MODEL, int NG>
 __global__ __launch_bounds__(WAVE * kMinfWaves) __attribute__((amdgpu_waves_per_eu(D <= 16 ? 3 : 1)))
 void k_minf_tile(
     double* __restrict__ partial, const float2* __restrict__ f, const float2* __restrict__ dir,
@@ -420,14 +420,14 @@ void k_minf_tile(
   float2* prl = reinterpret_cast<float2*>(lds_pd + EB * EB);
   float* T = reinterpret_cast<float*>(prl + ((P * P + 1) & ~1)) + wave * G * D * (D + 1);
   for (int i = threadIdx.x; i < P * P; i += blockDim.x) prl[i] = prb[i];
-  __shared__ float gsh[PT_MAX_GAM];
+  __shared__ float gsh[NG];
   const int tpa = nty * ntx, tpc = chunk * tpa;
   const long tile = (long)c * tpc + blockIdx.x;
   const int t = (int)(tile / tpa);
   const int rem = (int)(tile - (long)t * tpa);
   const int y0 = (rem / ntx) * B, x0 = (rem % ntx) * B;
   const int beg = tile_start[tile], end = tile_start[tile + 1];
-  if (threadIdx.x < PT_MAX_GAM)
+  if (threadIdx.x < NG)
     gsh[threadIdx.x] = threadIdx.x < ngam ? (float)gammas[(long)c * ngam + threadIdx.x] : 0.f;
   if (end > beg) {
     // all loads of the region first (one memory round trip), then the stores
@@ -450,9 +450,9 @@ void k_minf_tile(
     }
   }
   __syncthreads();
-  double acc[PT_MAX_GAM];
+  double acc[NG];  // candidate slots (NG >= ngam)
 #pragma unroll
-  for (int j = 0; j < PT_MAX_GAM; ++j) acc[j] = 0.0;
+  for (int j = 0; j < NG; ++j) acc[j] = 0.0;
   const int ix = r - ox;
   for (int i0 = beg + wave * G; i0 < end; i0 += kMinfWaves * G) {
     const int ii = i0 + slot;
@@ -489,18 +489,18 @@ void k_minf_tile(
       if (MODEL == PT_MODEL_GAUSSIAN) d[k] = sqrtf(d[k]);
     }
 #pragma unroll 1
-    for (int j = 0; j < PT_MAX_GAM; ++j) {
+    for (int j = 0; j < NG; ++j) {
       if (j < ngam && gsh[j] != 0.f) acc[0] += (double)delta_part<D, MODEL>(a, b, d, gsh[j]);
       const double t0 = acc[0];
 #pragma unroll
-      for (int q = 0; q < PT_MAX_GAM - 1; ++q) acc[q] = acc[q + 1];
-      acc[PT_MAX_GAM - 1] = t0;
+      for (int q = 0; q < NG - 1; ++q) acc[q] = acc[q + 1];
+      acc[NG - 1] = t0;
     }
   }
   const long nbt = (long)tpc * kMinfWaves;
   double* out = partial + (long)c * ngam * nbt + (long)blockIdx.x * kMinfWaves + wave;
 #pragma unroll
-  for (int j = 0; j < PT_MAX_GAM; ++j) {
+  for (int j = 0; j < NG; ++j) {
     if (j >= ngam) break;
     const double v = wave_sum(acc[j]);
     if (lane == 0) out[j * nbt] = v;
@@ -808,16 +808,21 @@ int run_minf(pt_ptycho* h, double* sums, const float2* f, const float2* dir, con
     }
     {
       ProfScope prof("ptycho_minf", st);
-      if (model == PT_MODEL_POISSON)
-        hipLaunchKernelGGL((k_minf_tile<D, PT_MODEL_POISSON>), dim3(tpc, nchunk),
-                           dim3(WAVE * kMinfWaves), tl, st, h->partial, f, dir, data, h->prb, h->pos,
-                           h->tile_start, h->tile_patch, gammas, ngam, chunk, h->nty, h->ntx, h->B,
-                           h->EB, h->nz, h->n, h->P, h->ox, h->oy, h->c, cp, cd, skip);
-      else
-        hipLaunchKernelGGL((k_minf_tile<D, PT_MODEL_GAUSSIAN>), dim3(tpc, nchunk),
-                           dim3(WAVE * kMinfWaves), tl, st, h->partial, f, dir, data, h->prb, h->pos,
-                           h->tile_start, h->tile_patch, gammas, ngam, chunk, h->nty, h->ntx, h->B,
-                           h->EB, h->nz, h->n, h->P, h->ox, h->oy, h->c, cp, cd, skip);
+      // the accumulator count is a template argument (registers): 5 slots cover
+      // the solver's sweeps (gamma = 0 and 4 halvings), 8 any other ngam
+      auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(tpc, nchunk), dim3(WAVE * kMinfWaves), tl, st, h->partial,
+                           f, dir, data, h->prb, h->pos, h->tile_start, h->tile_patch, gammas, ngam,
+                           chunk, h->nty, h->ntx, h->B, h->EB, h->nz, h->n, h->P, h->ox, h->oy,
+                           h->c, cp, cd, skip);
+      };
+      if (model == PT_MODEL_POISSON) {
+        if (ngam <= 5) launch(k_minf_tile<D, PT_MODEL_POISSON, 5>);
+        else launch(k_minf_tile<D, PT_MODEL_POISSON, PT_MAX_GAM>);
+      } else {
+        if (ngam <= 5) launch(k_minf_tile<D, PT_MODEL_GAUSSIAN, 5>);
+        else launch(k_minf_tile<D, PT_MODEL_GAUSSIAN, PT_MAX_GAM>);
+      }
     }
     PT_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_reduce_partials_t, dim3(nchunk, ngam), dim3(256), 0, st, sums, h->partial,
@@ -835,7 +840,7 @@ int run_minf(pt_ptycho* h, double* sums, const float2* f, const float2* dir, con
     h->partial_cap = need;
   }
   {
-  ProfScope prof("ptycho_minf", st);
+  ProfScope prof(lagr ? "ptycho_lagr" : "ptycho_minf", st);
   if (model == PT_MODEL_POISSON)
     hipLaunchKernelGGL((k_minf<D, PT_MODEL_POISSON>), dim3(nb, nchunk), dim3(WAVE), 0, st,
                        h->partial, f, dir, data, h->prb, h->pos, gammas, ngam, chunk, h->nz, h->n,
