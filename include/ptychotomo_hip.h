@@ -84,10 +84,11 @@ int pt_tomo_adj_scaled(pt_tomo* h, void* f, const void* g, float scale, pt_strea
 int pt_tomo_set_filter(pt_tomo* h, int kind);
 /* Launch shapes of the USFFT passes (no counterpart in the reference; set by
  * the autotuner): FFT lines per workgroup of the row pass (rows, two
- * N-point FFTs each), of the column passes (cols, dividing 2N) and of the
- * detector-line passes, forward inverse DFT and adjoint DFT (adj_lines <= 8,
- * (angle, slice) lines per workgroup).  Rejected if a shape does not fit
- * the LDS FFT; results do not depend on them. */
+ * N-point FFTs each), grid columns per workgroup of the column passes (cols,
+ * a power of two whose N x cols tile fits the per-thread registers) and
+ * (angle, slice) lines per workgroup of the detector-line passes, forward
+ * inverse DFT and adjoint DFT (adj_lines <= 8).  Rejected if a shape does
+ * not fit 64 KB of LDS; results do not depend on them. */
 int pt_tomo_get_tuning(pt_tomo* h, int* rows, int* cols, int* adj_lines);
 int pt_tomo_set_tuning(pt_tomo* h, int rows, int cols, int adj_lines);
 
