@@ -367,7 +367,7 @@ __global__ __launch_bounds__(64) void k_scatter_own(float2* __restrict__ Bg,
   constexpr int kSE = 32;
   __shared__ float wxs[kSE][17];
   __shared__ float wys[kSE][17];
-  __shared__ float2 gs[kSE][kZS + 1];
+  __shared__ float4 gs[kSE][kZS / 2 + 1];  // slice pairs (16-byte reads)
   __shared__ int eos[kSE];
   const int twoN = 2 * N;
   const int wi = blockIdx.x, z0 = blockIdx.y * kZS;
@@ -418,7 +418,8 @@ __global__ __launch_bounds__(64) void k_scatter_own(float2* __restrict__ Bg,
         eos[el] = on;
       } else {
 #pragma unroll
-        for (int j = 0; j < kZS; ++j) gs[el][j] = qn[j];
+        for (int j = 0; j < kZS / 2; ++j)
+          gs[el][j] = make_float4(qn[2 * j].x, qn[2 * j].y, qn[2 * j + 1].x, qn[2 * j + 1].y);
       }
     }
     __syncthreads();
@@ -426,18 +427,23 @@ __global__ __launch_bounds__(64) void k_scatter_own(float2* __restrict__ Bg,
     sc = sn;
     fetch_data(c0 + kSE, sc);
     sn = fetch_idx(c0 + 2 * kSE);
+    // branch-free: lanes outside the entry's footprint add w = 0 (exact), so
+    // the loop has uniform control flow and consecutive entries interleave
+#pragma unroll 2
     for (int e = 0; e < ne; ++e) {
       const int o = eos[e];
       const int k0 = lx - ((o & 255) - 64);
       const int k1 = ly - (((o >> 8) & 255) - 64);
-      if ((unsigned)k0 <= (unsigned)(2 * M) && (unsigned)k1 <= (unsigned)(2 * M)) {
-        const float w = wxs[e][k0] * wys[e][k1];
+      const bool in = (unsigned)k0 <= (unsigned)(2 * M) && (unsigned)k1 <= (unsigned)(2 * M);
+      const float wv = wxs[e][in ? k0 : 0] * wys[e][in ? k1 : 0];
+      const float w = in ? wv : 0.f;
 #pragma unroll
-        for (int j = 0; j < kZS; ++j) {
-          const float2 q = gs[e][j];
-          acc[j].x += w * q.x;
-          acc[j].y += w * q.y;
-        }
+      for (int j = 0; j < kZS / 2; ++j) {
+        const float4 q = gs[e][j];
+        acc[2 * j].x += w * q.x;
+        acc[2 * j].y += w * q.y;
+        acc[2 * j + 1].x += w * q.z;
+        acc[2 * j + 1].y += w * q.w;
       }
     }
   }
