@@ -62,26 +62,35 @@ def minf_work(history, nchunk, piter):
     `history` (one ADMM iteration per record).  A sweep evaluates every chunk
     whose search is still open (finished chunks exit at once), so the number
     of sweeps a chunk took follows from its accepted step: the search starts
-    at 2 (then at the previous step), sweep s tries halvings k = 4s .. 4s+3,
-    and a step reaching 1e-12 ends at 0.  Also returns the number of
+    at 2 (then at the previous step) and tries the halvings of the solver's
+    sweep schedule in order (the first CG step's schedule, then the later
+    steps'); a step reaching 1e-12 ends at 0.  Also returns the number of
     Lagrangian evaluations (launched under their own tag)."""
     import math
-    from ptychotomo_amd.solver import _PGAM, _PSWEEPS
-    per = _PGAM - 1
+    from ptychotomo_amd.solver import _PSCHED_FIRST, _PSCHED_LATER
+
+    def nsweeps(k, sched):
+        done = 0
+        for s, n in enumerate(sched):
+            done += n
+            if k < done:
+                return s + 1
+        return len(sched)
+
     sweeps, lagr = 0, 0
     for rec in history:
         if rec.get("lagr"):
             lagr += 1
         for chunk_g in rec["ptycho_gammas"]:
             g0 = 2.0
-            for g in chunk_g:
+            for i, g in enumerate(chunk_g):
                 if g0 <= 1e-12:
                     k = 0
                 elif g > 0:
                     k = int(round(math.log2(g0 / g)))
                 else:
                     k = math.ceil(math.log2(g0 / 1e-12))
-                sweeps += min(k // per + 1, _PSWEEPS)
+                sweeps += nsweeps(k, _PSCHED_FIRST if i == 0 else _PSCHED_LATER)
                 g0 = g
     return sweeps, lagr
 

@@ -268,16 +268,19 @@ int pt_k_dual(void* out, const void* lam, const void* a, const void* b, double c
  * 0 or not finite (after a zero step, where the reference divides 0 by 0). */
 int pt_k_dai_yuan(double* beta, const double* red, int nseg, pt_stream stream);
 /* Halving line search (solver.py:171-177) per segment as sweeps of ngam
- * candidates.  state [nseg][2] = {current step, done}: begin writes
- * cand[s][0] = 0, cand[s][j] = cur*2^-(j-1) and clears done.  After the
- * functional f = w[0]*f1 + w[1]*f2 (f2 may be NULL; w: 2 device FP64) is
+ * candidates (slot 0 is gamma = 0; sweeps may differ in width).  state
+ * [nseg][2] = {current step, done}: begin writes cand[s][0] = 0,
+ * cand[s][j] = cur*2^-(j-1) (ngam slots per segment) and clears done.  After
+ * the functional f = w[0]*f1 + w[1]*f2 (f2 may be NULL; w: 2 device FP64) is
  * evaluated at cand, select accepts per segment the first candidate that is
  * not a strict ascent or is <= 1e-12 (then 0), writing step[s] = {gamma, 0}
- * and state = {gamma, 1}; unfinished segments get the next sweep's
- * candidates; with last != 0 they end at 0. */
+ * and state = {gamma, 1}; unfinished segments get the next sweep's ngam_next
+ * candidates in cand_next (a second buffer when ngam_next != ngam and
+ * nseg > 1); with last != 0 they end at 0. */
 int pt_k_ls_begin(double* state, double* cand, int nseg, int ngam, pt_stream stream);
-int pt_k_ls_select(double* state, double* cand, double* step, const double* f1, const double* f2,
-                   const double* w, int nseg, int ngam, int last, pt_stream stream);
+int pt_k_ls_select(double* state, double* cand, double* cand_next, double* step, const double* f1,
+                   const double* f2, const double* w, int nseg, int ngam, int ngam_next, int last,
+                   pt_stream stream);
 /* Residual balancing (solver.py:267-282): rho_tau = {rho, tau} updated in
  * place from norms = {||psi-h||^2, ||h-h0||^2, ||phi-e||^2, ||e-e0||^2}. */
 int pt_k_penalty(double* rho_tau, const double* norms, pt_stream stream);

@@ -86,7 +86,7 @@ _SIGS = {
     "pt_k_dual": (c_int, [vp, vp, vp, vp, c_double, vp, c_int, c_long, vp]),
     "pt_k_dai_yuan": (c_int, [vp, vp, c_int, vp]),
     "pt_k_ls_begin": (c_int, [vp, vp, c_int, c_int, vp]),
-    "pt_k_ls_select": (c_int, [vp, vp, vp, vp, vp, vp, c_int, c_int, c_int, vp]),
+    "pt_k_ls_select": (c_int, [vp, vp, vp, vp, vp, vp, vp, c_int, c_int, c_int, c_int, vp]),
     "pt_k_penalty": (c_int, [vp, vp, vp]),
     "pt_nn_conv3d": (c_int, [vp, vp, vp, vp, c_int, c_int, c_int, c_int, c_int, c_int, vp]),
     "pt_nn_conv3d_wflip": (c_int, [vp, vp, c_int, c_int, vp]),

@@ -422,13 +422,14 @@ void k_minf_tile(
   for (int i = threadIdx.x; i < P * P; i += blockDim.x) prl[i] = prb[i];
   __shared__ float gsh[NG];
   const int tpa = nty * ntx, tpc = chunk * tpa;
-  const long tile = (long)c * tpc + blockIdx.x;
+  if (threadIdx.x < NG)
+    gsh[threadIdx.x] = threadIdx.x < ngam ? (float)gammas[(long)c * ngam + threadIdx.x] : 0.f;
+  const int tb = blockIdx.x;
+  const long tile = (long)c * tpc + tb;
   const int t = (int)(tile / tpa);
   const int rem = (int)(tile - (long)t * tpa);
   const int y0 = (rem / ntx) * B, x0 = (rem % ntx) * B;
   const int beg = tile_start[tile], end = tile_start[tile + 1];
-  if (threadIdx.x < NG)
-    gsh[threadIdx.x] = threadIdx.x < ngam ? (float)gammas[(long)c * ngam + threadIdx.x] : 0.f;
   if (end > beg) {
     // all loads of the region first (one memory round trip), then the stores
     const float2* fz = f + (long)t * nz * n;
@@ -498,7 +499,7 @@ void k_minf_tile(
     }
   }
   const long nbt = (long)tpc * kMinfWaves;
-  double* out = partial + (long)c * ngam * nbt + (long)blockIdx.x * kMinfWaves + wave;
+  double* out = partial + (long)c * ngam * nbt + (long)tb * kMinfWaves + wave;
 #pragma unroll
   for (int j = 0; j < NG; ++j) {
     if (j >= ngam) break;

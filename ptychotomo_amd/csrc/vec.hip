@@ -52,7 +52,7 @@ namespace {
 
 constexpr int RB = 256;          // threads per reduction block
 constexpr int RMAXBLK = 8192;    // blocks per segment cap
-constexpr int RMAXTOT = 16384;   // blocks per reduction cap (all segments)
+constexpr int RMAXTOT = 4096;    // blocks per segmented reduction cap (all segments)
 
 // scratch for partial sums, one per stream (grown on demand): reductions
 // enqueued on different streams may run concurrently
@@ -118,7 +118,7 @@ int seg_reduce(double* out, long len, int nseg, F f, hipStream_t st) {
   PT_REQUIRE(nseg > 0 && len % nseg == 0, PT_ERR_SIZE, "reduction: nseg must divide len");
   const long seglen = len / nseg;
   int nblk = (int)std::min<long>(RMAXBLK, std::max<long>(1, (seglen + RB * 4 - 1) / (RB * 4)));
-  nblk = std::max(1, std::min(nblk, std::max(1, RMAXTOT / nseg)));
+  nblk = std::max(1, std::min(nblk, std::max(1, (nseg == 1 ? RMAXBLK : RMAXTOT) / nseg)));
   double* part;
   int rc = partial_buf((size_t)nseg * nblk * NV, &part, st);
   if (rc) return rc;
@@ -140,7 +140,7 @@ int seg_reduce_n(double* out, long len, int nseg, int nout, F f, hipStream_t st,
   PT_REQUIRE(nseg > 0 && len % nseg == 0, PT_ERR_SIZE, "reduction: nseg must divide len");
   const long seglen = len / nseg;
   int nblk = (int)std::min<long>(RMAXBLK, std::max<long>(1, (seglen + RB * 4 - 1) / (RB * 4)));
-  nblk = std::max(1, std::min(nblk, std::max(1, RMAXTOT / nseg)));
+  nblk = std::max(1, std::min(nblk, std::max(1, (nseg == 1 ? RMAXBLK : RMAXTOT) / nseg)));
   double* part;
   int rc = partial_buf((size_t)nseg * nblk * NV, &part, st);
   if (rc) return rc;
@@ -971,12 +971,13 @@ __global__ void k_ls_begin(double* __restrict__ state, double* __restrict__ cand
 
 // f[s][j] = w[0]*f1[s][j] + w[1]*f2[s][j] (f2 may be null).  A segment accepts
 // the first candidate that is not a strict ascent or is <= 1e-12 (then 0);
-// otherwise its next sweep continues from half the last candidate.  With
-// `last`, an unfinished segment ends at 0 as the halving rule would.
-__global__ void k_ls_select(double* __restrict__ state, double* __restrict__ cand,
+// otherwise its next sweep (ngam_next slots, written to cand_next) continues
+// from half the last candidate.  With `last`, an unfinished segment ends at 0
+// as the halving rule would.
+__global__ void k_ls_select(double* __restrict__ state, const double* cand, double* cand_next,
                             double* __restrict__ step, const double* __restrict__ f1,
                             const double* __restrict__ f2, const double* __restrict__ w, int nseg,
-                            int ngam, int last) {
+                            int ngam, int ngam_next, int last) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nseg) return;
   if (state[2 * s + 1] != 0.0) return;
@@ -1003,8 +1004,8 @@ __global__ void k_ls_select(double* __restrict__ state, double* __restrict__ can
     return;
   }
   state[2 * s] = cur;
-  cand[s * ngam] = 0.0;
-  for (int j = 1; j < ngam; ++j) cand[s * ngam + j] = cur * ldexp(1.0, -(j - 1));
+  cand_next[s * ngam_next] = 0.0;
+  for (int j = 1; j < ngam_next; ++j) cand_next[s * ngam_next + j] = cur * ldexp(1.0, -(j - 1));
 }
 
 // Residual balancing (solver.py:267-282).  norms = {||psi - h||^2,
@@ -1042,14 +1043,17 @@ extern "C" int pt_k_ls_begin(double* state, double* cand, int nseg, int ngam, pt
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
-extern "C" int pt_k_ls_select(double* state, double* cand, double* step, const double* f1,
-                              const double* f2, const double* w, int nseg, int ngam, int last,
-                              pt_stream stream) {
-  PT_REQUIRE(state && cand && step && f1 && w && nseg > 0, PT_ERR_ARG,
+extern "C" int pt_k_ls_select(double* state, double* cand, double* cand_next, double* step,
+                              const double* f1, const double* f2, const double* w, int nseg,
+                              int ngam, int ngam_next, int last, pt_stream stream) {
+  PT_REQUIRE(state && cand && cand_next && step && f1 && w && nseg > 0, PT_ERR_ARG,
              "pt_k_ls_select: bad argument");
-  PT_REQUIRE(ngam >= 2 && ngam <= PT_MAX_CAND, PT_ERR_ARG, "pt_k_ls_select: 2 <= ngam <= 8");
+  PT_REQUIRE(ngam >= 2 && ngam <= PT_MAX_CAND && ngam_next >= 2 && ngam_next <= PT_MAX_CAND,
+             PT_ERR_ARG, "pt_k_ls_select: 2 <= ngam, ngam_next <= 8");
+  PT_REQUIRE(cand_next != cand || ngam_next == ngam || nseg == 1, PT_ERR_ARG,
+             "pt_k_ls_select: a sweep of another width needs its own candidate buffer");
   hipLaunchKernelGGL(k_ls_select, dim3((nseg + 63) / 64), dim3(64), 0, (hipStream_t)stream, state,
-                     cand, step, f1, f2, w, nseg, ngam, last);
+                     cand, cand_next, step, f1, f2, w, nseg, ngam, ngam_next, last);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }

@@ -41,10 +41,16 @@ from .radonusfft import radonusfft
 PLANCK_CONSTANT = 6.58211928e-19  # [keV*s]   solver.py:15
 SPEED_OF_LIGHT = 299792458e+2     # [cm/s]    solver.py:16
 
-_MAXGAM = 8      # cg_tomo line-search candidates per sweep (gamma = 0 and 7 halvings)
-_SWEEPS = 6      # its sweeps per line search: 42 halvings take 2 below 1e-12
-_PGAM = 5        # cg_ptycho: gamma = 0 and 4 halvings per sweep (one sweep covers the
-_PSWEEPS = 11    # usual 0-3 halvings; 44 halvings take 2 below 1e-12)
+_MAXGAM = 8      # candidate slots of the line-search buffers (gamma = 0 + 7 halvings)
+# halvings per sweep of the device line searches; every schedule covers at least
+# 41 halvings (2 -> below 1e-12), so the halving rule is followed to its end.
+# cg_tomo's evaluations are cheap: 7 per sweep.  cg_ptycho's first CG step starts
+# at gamma = 2 and usually accepts after 3-4 halvings (one sweep of 5); the later
+# steps start from the previous step and usually accept it or its half (sweeps
+# of 2, 2, 4, then 7)
+_TSCHED = (7,) * 6
+_PSCHED_FIRST = (5,) + (7,) * 6
+_PSCHED_LATER = (2, 2, 4) + (7,) * 5
 # device scalar block (FP64): 1, rho, tau, takexi {pshift, cos, sin, kfac},
 # takexi statistics, penalty norms, max |angle psi|
 _ONE, _RHO, _TAU, _PRM, _STATS, _NORMS, _MAXANG, _NSC = 0, 1, 2, 3, 8, 12, 16, 24
@@ -266,7 +272,8 @@ class Solver(object):
             z = lambda n: torch.zeros(n, dtype=f64, device=device())
             w = {"shape": tuple(shape), "nseg": nseg,
                  "red": z(3 * nseg), "beta": z(2 * nseg), "state": z(2 * nseg),
-                 "step": z(2 * nseg), "cand": z(_MAXGAM * nseg), "f1": z(_MAXGAM * nseg),
+                 "step": z(2 * nseg), "cand": z(_MAXGAM * nseg), "cand2": z(_MAXGAM * nseg),
+                 "f1": z(_MAXGAM * nseg),
                  "f2": z(_MAXGAM * nseg), "sc": z(4),
                  "state0": torch.tensor([2.0, 0.0] * nseg, dtype=f64, device=device()),
                  "grad": [empty(shape), empty(shape)], "d": empty(shape)}
@@ -306,19 +313,26 @@ class Solver(object):
             gamma = 0
         return gamma
 
-    def _line_search_dev(self, w, nseg, evalf, weights, ngam=_MAXGAM, sweeps=_SWEEPS):
+    def _line_search_dev(self, w, nseg, evalf, weights, sched=_TSCHED):
         """The halving rule of line_search for nseg independent segments, on
-        the device: sweeps of ngam - 1 halvings (plus gamma = 0) until every
-        segment has accepted a step (`sweeps` sweeps cover 2 -> 1e-12).
-        evalf(cand, state, f1, f2) enqueues the two functional terms (their
-        change from gamma = 0) at the candidates, skipping finished segments;
-        weights points to their two FP64 weights."""
+        the device: sweeps of sched[i] halvings (plus gamma = 0) until every
+        segment has accepted a step (a schedule covers 2 -> 1e-12).
+        evalf(cand, ngam, state, f1, f2) enqueues the two functional terms
+        (their change from gamma = 0) at the ngam candidate slots per segment,
+        skipping finished segments; weights points to their two FP64 weights.
+        Consecutive sweeps alternate between two candidate buffers."""
         st = stream()
-        _call("pt_k_ls_begin", ptr(w["state"]), ptr(w["cand"]), nseg, ngam, st)
-        for sw in range(sweeps):
-            evalf(w["cand"], w["state"], w["f1"], w["f2"])
-            _call("pt_k_ls_select", ptr(w["state"]), ptr(w["cand"]), ptr(w["step"]),
-                  ptr(w["f1"]), ptr(w["f2"]), weights, nseg, ngam, int(sw == sweeps - 1), st)
+        bufs = (w["cand"], w["cand2"])
+        ng = sched[0] + 1
+        _call("pt_k_ls_begin", ptr(w["state"]), ptr(bufs[0]), nseg, ng, st)
+        for sw in range(len(sched)):
+            last = sw == len(sched) - 1
+            ng_next = ng if last else sched[sw + 1] + 1
+            cur, nxt = bufs[sw % 2], bufs[(sw + 1) % 2]
+            evalf(cur, ng, w["state"], w["f1"], w["f2"])
+            _call("pt_k_ls_select", ptr(w["state"]), ptr(cur), ptr(nxt), ptr(w["step"]),
+                  ptr(w["f1"]), ptr(w["f2"]), weights, nseg, ng, ng_next, int(last), st)
+            ng = ng_next
 
     def _dai_yuan_dev(self, w, i, grad, grad0, L, nseg):
         """d = -grad (i = 0) or -grad + beta d (solver.py:192-196, 227-231)."""
@@ -370,10 +384,10 @@ class Solver(object):
             grad0 = grad
             self._tomo.fwd_scaled(Rd.data_ptr(), d.data_ptr(), ct)
 
-            def evalf(cand, state, f1, f2):
+            def evalf(cand, ng, state, f1, f2):
                 _call("pt_k_tomo_minf", ptr(f1), ptr(K), ptr(Ru), ptr(Rd), ptr(xi0), ptr(cand),
-                      _MAXGAM, lenp, ptr(state), st)
-                _call("pt_k_reg_minf", ptr(f2), ptr(u), ptr(d), ptr(xi1), ptr(cand), _MAXGAM,
+                      ng, lenp, ptr(state), st)
+                _call("pt_k_reg_minf", ptr(f2), ptr(u), ptr(d), ptr(xi1), ptr(cand), ng,
                       nz, n, n, ptr(state), st)
 
             # minf = rho*|K R u - xi0|^2 + tau*|J u - xi1|^2 (solver.py:181-182)
@@ -430,14 +444,15 @@ class Solver(object):
             self._dai_yuan_dev(w, i, grad, grad0, L, nseg)
             grad0 = grad
 
-            def evalf(cand, state, f1, f2):
+            def evalf(cand, ng, state, f1, f2):
                 _call("pt_ptycho_minf", op._h, ptr(f1), ptr(psi), ptr(d), ptr(data), ptr(cand),
-                      _PGAM, chunk, cp, cd, mdl, 0, ptr(state), st)
+                      ng, chunk, cp, cd, mdl, 0, ptr(state), st)
                 _call("pt_k_rho_minf", ptr(f2), ptr(h), ptr(psi), ptr(lamd), ptr(d), 0.0,
-                      _dp(blk, _RHO), ptr(cand), _PGAM, L, nseg, ptr(state), st)
+                      _dp(blk, _RHO), ptr(cand), ng, L, nseg, ptr(state), st)
 
             # minf = data term + rho*|h - psi + lamd/rho|^2 (solver.py:208-213)
-            self._line_search_dev(w, nseg, evalf, _dp(blk, _ONE), _PGAM, _PSWEEPS)
+            self._line_search_dev(w, nseg, evalf, _dp(blk, _ONE),
+                                  _PSCHED_FIRST if i == 0 else _PSCHED_LATER)
             _call("pt_k_axpy_seg", ptr(psi), ptr(w["step"]), ptr(d), L, nseg, st)
             if hist is not None:
                 hist[i].copy_(w["step"][0::2])
