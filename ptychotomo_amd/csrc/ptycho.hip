@@ -809,8 +809,9 @@ int run_minf(pt_ptycho* h, double* sums, const float2* f, const float2* dir, con
     }
     {
       ProfScope prof("ptycho_minf", st);
-      // the accumulator count is a template argument (registers): 5 slots cover
-      // the solver's sweeps (gamma = 0 and 4 halvings), 8 any other ngam
+      // the accumulator count is a template argument (registers): 3, 5 and 6
+      // slots cover the solver's sweeps (gamma = 0 and 2, 4 or 5 halvings), 8
+      // any other ngam
       auto launch = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(tpc, nchunk), dim3(WAVE * kMinfWaves), tl, st, h->partial,
                            f, dir, data, h->prb, h->pos, h->tile_start, h->tile_patch, gammas, ngam,
@@ -818,10 +819,14 @@ int run_minf(pt_ptycho* h, double* sums, const float2* f, const float2* dir, con
                            h->c, cp, cd, skip);
       };
       if (model == PT_MODEL_POISSON) {
-        if (ngam <= 5) launch(k_minf_tile<D, PT_MODEL_POISSON, 5>);
+        if (ngam <= 3) launch(k_minf_tile<D, PT_MODEL_POISSON, 3>);
+        else if (ngam <= 5) launch(k_minf_tile<D, PT_MODEL_POISSON, 5>);
+        else if (ngam <= 6) launch(k_minf_tile<D, PT_MODEL_POISSON, 6>);
         else launch(k_minf_tile<D, PT_MODEL_POISSON, PT_MAX_GAM>);
       } else {
-        if (ngam <= 5) launch(k_minf_tile<D, PT_MODEL_GAUSSIAN, 5>);
+        if (ngam <= 3) launch(k_minf_tile<D, PT_MODEL_GAUSSIAN, 3>);
+        else if (ngam <= 5) launch(k_minf_tile<D, PT_MODEL_GAUSSIAN, 5>);
+        else if (ngam <= 6) launch(k_minf_tile<D, PT_MODEL_GAUSSIAN, 6>);
         else launch(k_minf_tile<D, PT_MODEL_GAUSSIAN, PT_MAX_GAM>);
       }
     }
