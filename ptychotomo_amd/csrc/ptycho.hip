@@ -300,7 +300,7 @@ __device__ __forceinline__ float data_term(const float2 (&a)[D], const float (&d
 // LDS footprint is the transpose buffer only.
 #define PT_MAX_GAM 8
 template <int D, int MODEL>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(D <= 16 ? 3 : 1)))
 void k_minf(double* __restrict__ partial,
                                                const float2* __restrict__ f,
                                                const float2* __restrict__ dir,
@@ -794,9 +794,13 @@ int run_minf(pt_ptycho* h, double* sums, const float2* f, const float2* dir, con
              const double* skip, hipStream_t st) {
   constexpr int G = WAVE / D;
   const int nchunk = h->pt / chunk;
-  // line-search sums over the overlap-add tiles (frames read from LDS)
+  // line-search sums over the overlap-add tiles (frames read from LDS), where
+  // a tile holds on average at least one patch per slot of its workgroup;
+  // sparser tiles (e.g. 32 x 32 frames at step 8: B = 8, one patch per tile)
+  // would leave most lanes idle, and their patches read their frames directly
   const size_t tl = minf_tile_lds_bytes(D, h->EB, h->P);
-  if constexpr (D <= 32) if (!lagr && dir && tl <= 64 * 1024 &&
+  const bool dense = (long)h->S >= (long)kMinfWaves * G * h->nty * h->ntx;
+  if constexpr (D <= 32) if (!lagr && dir && dense && tl <= 64 * 1024 &&
                             h->EB * h->EB <= WAVE * kMinfWaves * kMinfStage) {
     const int tpc = chunk * h->nty * h->ntx;
     const int nbt = tpc * kMinfWaves;
@@ -933,7 +937,6 @@ extern "C" int pt_ptycho_create(int ptheta, int nz, int n, int ntheta0, int nsca
   h->ox = (detx - nprb) / 2;
   h->oy = (dety - nprb) / 2;
   h->c = 1.0f / sqrtf((float)(detx * dety));
-  // tile size from the LDS budget
   // tile size from the LDS budget: the extended tile takes what the
   // transpose buffers leave, and at least P+7 pixels
   const size_t tbytes = tile_lds_bytes(detx, 0);

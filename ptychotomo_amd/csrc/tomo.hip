@@ -431,7 +431,9 @@ __global__ __launch_bounds__(64) void k_scatter_own(float2* __restrict__ Bg,
     // the loop has uniform control flow and consecutive entries interleave
 #pragma unroll 2
     for (int e = 0; e < ne; ++e) {
-      const int o = eos[e];
+      // the entry's footprint offset is the same for all lanes: decode it on
+      // the scalar unit
+      const int o = __builtin_amdgcn_readfirstlane(eos[e]);
       const int k0 = lx - ((o & 255) - 64);
       const int k1 = ly - (((o >> 8) & 255) - 64);
       const bool in = (unsigned)k0 <= (unsigned)(2 * M) && (unsigned)k1 <= (unsigned)(2 * M);
