@@ -10,9 +10,11 @@ iterations (including the two piter+32 iterations of the reference
 schedule) precede K timed iterations.
 
 Prints one JSON line with the headline number, the roofline of the dominant
-kernel (HIP events around each of its launches inside the timed region) and
-the CPU baseline (the NumPy restatement of the reference operators on a
-bounded sample, run on this host's cores).
+kernel (HIP events around each of its launches inside the timed region), the
+whole-volume operators' rates (tomography and ptychography forward/adjoint,
+GB/s against HBM, and against the same operators on the host) and the CPU
+baseline (the NumPy restatement of the reference operators on a bounded
+sample, run on this host's cores).
 """
 import argparse
 import ctypes
@@ -160,6 +162,51 @@ def cpu_baseline(w, budget_s=30.0):
             "seconds_per_iter": per_iter,
             "op_seconds": {"usfft_fwd": t_tf, "usfft_adj": t_ta, "ptycho_fwd": t_pf,
                            "ptycho_adj": t_pa, "ptycho_residual": t_pr}}
+
+
+def operator_rates(slv, reps=5):
+    """The workload's whole-volume operators as the solver calls them
+    (tomography forward and adjoint, ptychography forward to the far field
+    and adjoint, scaled by coeftomo / coefptycho): median of `reps` calls
+    after one untimed call, HIP events on the launch stream; algorithmic
+    bytes = the input read and the output written once."""
+    import torch
+    n, nz, nt = slv.n, slv.nz, slv.ntheta
+    S, D = slv.nscan, slv.ndety
+    c64 = torch.complex64
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    u = torch.randn((nz, n, n), dtype=c64, device="cuda", generator=gen)
+    g = torch.randn((nt, nz, n), dtype=c64, device="cuda", generator=gen)
+    u2, g2 = torch.empty_like(u), torch.empty_like(g)
+    far = torch.empty((nt, S, D, D), dtype=c64, device="cuda")
+    ct, cp = float(slv.coeftomo), float(slv.coefptycho)
+    tomo_b = 8 * (nz * n * n + nt * nz * n)
+    ptycho_b = 8 * nt * nz * n + 8 * nt * S * D * D
+    ops = {
+        "tomo_fwd": (lambda: slv._tomo.fwd_scaled(g2.data_ptr(), u.data_ptr(), ct), tomo_b),
+        "tomo_adj": (lambda: slv._tomo.adj_scaled(u2.data_ptr(), g.data_ptr(), ct), tomo_b),
+        "ptycho_fwd": (lambda: slv._ptycho.fwd_scaled(far.data_ptr(), g.data_ptr(), cp),
+                       ptycho_b),
+        "ptycho_adj": (lambda: slv._ptycho.adj_scaled(g2.data_ptr(), far.data_ptr(), cp),
+                       ptycho_b),
+    }
+    out = {}
+    for name, (fn, nbytes) in ops.items():
+        fn()
+        ms = []
+        for _ in range(reps):
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record()
+            fn()
+            ev[1].record()
+            ev[1].synchronize()
+            ms.append(ev[0].elapsed_time(ev[1]))
+        t = float(np.median(ms))
+        out[name] = {"ms": t, "algorithmic_bytes": nbytes, "GB_s": nbytes / (t * 1e-3) / 1e9,
+                     "frac_hbm": nbytes / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    del u, g, u2, g2, far
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -311,6 +358,8 @@ def main():
     lag = [r.get("lagr") for r in slv.history if r.get("lagr")]
     if lag:
         out["lagr_last"] = lag[-1]
+    log("operator rates")
+    out["operators"] = operator_rates(slv)
     if args.tol_run:
         out["time_to_tol"] = time_to_tol(slv, data, w, run, args.tol_run)
     if args.warm_weights and not args.no_warm and os.path.exists(args.warm_weights):
@@ -337,6 +386,13 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(w)
+        # the same operators on the host (whole-volume times scaled from the
+        # bounded sample)
+        cs = out["cpu_baseline"]["op_seconds"]
+        for name, key in (("tomo_fwd", "usfft_fwd"), ("tomo_adj", "usfft_adj"),
+                          ("ptycho_fwd", "ptycho_fwd"), ("ptycho_adj", "ptycho_adj")):
+            out["operators"][name]["cpu_ms"] = cs[key] * 1e3
+            out["operators"][name]["vs_cpu"] = cs[key] * 1e3 / out["operators"][name]["ms"]
     print(json.dumps(out), flush=True)
 
 
