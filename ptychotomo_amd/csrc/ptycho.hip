@@ -794,13 +794,14 @@ int run_minf(pt_ptycho* h, double* sums, const float2* f, const float2* dir, con
              const double* skip, hipStream_t st) {
   constexpr int G = WAVE / D;
   const int nchunk = h->pt / chunk;
-  // line-search sums over the overlap-add tiles (frames read from LDS), where
-  // a tile holds on average at least one patch per slot of its workgroup;
-  // sparser tiles (e.g. 32 x 32 frames at step 8: B = 8, one patch per tile)
-  // would leave most lanes idle, and their patches read their frames directly
+  // line-search sums over the overlap-add tiles (frames read from LDS) for
+  // frames up to 16 x 16 whose extended tile fits the staging, where a tile
+  // holds on average at least one patch per slot of its workgroup (sparser
+  // tiles would leave lanes idle); otherwise each patch reads its frame rows
+  // directly (32 x 32 frames: 63 x 63 extended tiles)
   const size_t tl = minf_tile_lds_bytes(D, h->EB, h->P);
   const bool dense = (long)h->S >= (long)kMinfWaves * G * h->nty * h->ntx;
-  if constexpr (D <= 32) if (!lagr && dir && dense && tl <= 64 * 1024 &&
+  if constexpr (D <= 16) if (!lagr && dir && dense && tl <= 64 * 1024 &&
                             h->EB * h->EB <= WAVE * kMinfWaves * kMinfStage) {
     const int tpc = chunk * h->nty * h->ntx;
     const int nbt = tpc * kMinfWaves;
@@ -911,9 +912,15 @@ int tiles_impl(pt_ptycho* h, float2* out, const float2* gin, const float2* f, co
   return run_fold<MODE>(h, out, f, hh, lamd, (float)rho, rho_dev, st);
 }
 
-// LDS budget of an overlap-add workgroup (13 resident per CU at 12 KB; for a
-// 16x16 probe the tile is 16 x 16 pixels, extended to 31 x 31)
+// LDS budget of an overlap-add workgroup.  12 KB for frames up to 16 x 16
+// (13 resident per CU; a 16x16 probe gets 16 x 16 pixel tiles, extended to
+// 31 x 31).  40 KB for larger frames: at 32 x 32 the transpose buffers alone
+// take 8.4 KB, and in 12 KB the tiles would shrink to 8 pixels -- one patch
+// per tile at step 8 (half of the wave's frame slots idle) and ~25
+// overlapping tiles summed per pixel by the fold; at 40 KB a 32x32 probe gets
+// 32 x 32 pixel tiles extended to 63 x 63.
 constexpr size_t kTileLdsBudget = 12 * 1024;
+constexpr size_t kTileLdsBudgetLarge = 40 * 1024;
 
 }  // namespace
 }  // namespace pt
@@ -940,7 +947,8 @@ extern "C" int pt_ptycho_create(int ptheta, int nz, int n, int ntheta0, int nsca
   // tile size from the LDS budget: the extended tile takes what the
   // transpose buffers leave, and at least P+7 pixels
   const size_t tbytes = tile_lds_bytes(detx, 0);
-  const size_t left = kTileLdsBudget > tbytes + 4096 ? kTileLdsBudget - tbytes : 4096;
+  const size_t budget = detx >= 32 ? kTileLdsBudgetLarge : kTileLdsBudget;
+  const size_t left = budget > tbytes + 4096 ? budget - tbytes : 4096;
   const long maxe = (long)std::sqrt((double)left / sizeof(float2));
   int EB = (int)std::max<long>(maxe, nprb + 7);
   int B = EB - (nprb - 1);

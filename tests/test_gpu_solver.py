@@ -42,9 +42,10 @@ CFGS = {"d8": dict(n=32, nz=32, ntheta=24, nprb=8, det=8, step=4, ptheta=8, pnz=
 
 
 # 32 x 32 frames (the 512^3 workload's probe): the gradient and the
-# line-search sums only.  At step 8 the overlap-add tiles (8 pixels) hold
-# one patch each and the sums run on the per-patch kernel; at step 2 a tile
-# holds nine and they run on the staged tile kernel.
+# line-search sums only (per-patch kernel).  The overlap-add tiles are 32
+# pixels wide (extended to 63): at step 8 on a 64-pixel object 2 x 2 tiles
+# of up to 4 patches, at step 2 on a 48-pixel object one tile of 81 patches
+# whose extended region reaches past the object.
 MINF_CFGS = dict(CFGS, d32=dict(n=64, nz=64, ntheta=8, nprb=32, det=32, step=8, ptheta=4,
                                 pnz=16, nell=6, photons=1e4),
                  d32s2=dict(n=48, nz=48, ntheta=4, nprb=32, det=32, step=2, ptheta=2,
