@@ -232,6 +232,8 @@ def main():
     ap.add_argument("--tol-iters", type=int, default=100)
     ap.add_argument("--tol-seeds", type=int, default=3, help="held-out phantoms compared")
     ap.add_argument("--no-warm", action="store_true", help="skip the plain-vs-warm comparison")
+    ap.add_argument("--no-operators", action="store_true",
+                    help="skip the whole-volume operator rates (profiled runs of the iteration)")
     ap.add_argument("--kernel", default=None,
                     help="roofline kernel tag (default: the largest of the breakdown probe; "
                          "ptycho_minf without the probe)")
@@ -358,8 +360,9 @@ def main():
     lag = [r.get("lagr") for r in slv.history if r.get("lagr")]
     if lag:
         out["lagr_last"] = lag[-1]
-    log("operator rates")
-    out["operators"] = operator_rates(slv)
+    if not args.no_operators:
+        log("operator rates")
+        out["operators"] = operator_rates(slv)
     if args.tol_run:
         out["time_to_tol"] = time_to_tol(slv, data, w, run, args.tol_run)
     if args.warm_weights and not args.no_warm and os.path.exists(args.warm_weights):
@@ -389,8 +392,9 @@ def main():
         # the same operators on the host (whole-volume times scaled from the
         # bounded sample)
         cs = out["cpu_baseline"]["op_seconds"]
-        for name, key in (("tomo_fwd", "usfft_fwd"), ("tomo_adj", "usfft_adj"),
-                          ("ptycho_fwd", "ptycho_fwd"), ("ptycho_adj", "ptycho_adj")):
+        for name, key in (() if args.no_operators else
+                          (("tomo_fwd", "usfft_fwd"), ("tomo_adj", "usfft_adj"),
+                           ("ptycho_fwd", "ptycho_fwd"), ("ptycho_adj", "ptycho_adj"))):
             out["operators"][name]["cpu_ms"] = cs[key] * 1e3
             out["operators"][name]["vs_cpu"] = cs[key] * 1e3 / out["operators"][name]["ms"]
     print(json.dumps(out), flush=True)
