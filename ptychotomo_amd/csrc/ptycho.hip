@@ -52,6 +52,20 @@ constexpr int WAVE = 64;
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (linear id w runs on
+// XCD w % 8), each with its own L2.  xcd_contig renumbers linear ids so that
+// every XCD runs one contiguous range of logical ids (the first total % 8
+// XCDs one id more than the rest), in dispatch order; a bijection on
+// [0, total), results do not depend on it.  Used by the overlap-add tiles
+// (adjacent tiles' extended regions overlap: 5.9 -> 5.6 ms per 256^3
+// iteration); the line-search and forward kernels measured slower with it
+// (line search 12.5 -> 15.7 ms per 256^3 iteration) and keep the hardware
+// order.
+__device__ __forceinline__ int xcd_contig(int w, int total) {
+  const int q = total >> 3, r = total & 7, x = w & 7;
+  return x * q + min(x, r) + (w >> 3);
+}
+
 // ------------------------------------------------------------------ transforms
 // Forward 2D transform of the frame rows held by the wave (v = row r of the
 // lane's patch) -> lane c holds far-field column c (in v).
@@ -314,7 +328,7 @@ void k_minf(double* __restrict__ partial,
   constexpr int G = WAVE / D;
   __shared__ float T[G * D * (D + 1)];
   const int lane = lane_id(), slot = lane / D, r = lane % D;
-  const int c = blockIdx.y;
+  const int c = blockIdx.y, bx = blockIdx.x;
   // chunks whose line search has finished (skip[2*c+1] != 0) are not evaluated
   if (skip && skip[2 * c + 1] != 0.0) return;
   __shared__ float gsh[PT_MAX_GAM];
@@ -325,7 +339,7 @@ void k_minf(double* __restrict__ partial,
   for (int j = 0; j < PT_MAX_GAM; ++j) acc[j] = 0.0;
   const long npatch = (long)chunk * S;
   const long base = (long)c * chunk * S;
-  for (long grp = blockIdx.x; grp * G < npatch; grp += gridDim.x) {
+  for (long grp = bx; grp * G < npatch; grp += gridDim.x) {
     const long pl = grp * G + slot;
     const bool active = pl < npatch;
     const long p = base + pl;
@@ -389,7 +403,7 @@ void k_minf(double* __restrict__ partial,
   for (int j = 0; j < PT_MAX_GAM; ++j) {
     if (j >= ngam) break;
     const double v = wave_sum(acc[j]);
-    if (lane == 0) partial[((long)c * gridDim.x + blockIdx.x) * ngam + j] = v;
+    if (lane == 0) partial[((long)c * gridDim.x + bx) * ngam + j] = v;
   }
 }
 
@@ -569,7 +583,7 @@ __global__ __launch_bounds__(WAVE * tile_waves<D>()) void k_tile(
   float2* acc = reinterpret_cast<float2*>(lds_f);          // EB*EB
   const int wave = threadIdx.x / WAVE;
   float* T = lds_f + 2 * EB * EB + wave * G * D * (D + 1);  // this wave's buffer
-  const long tile = blockIdx.x;
+  const long tile = xcd_contig(blockIdx.x, gridDim.x);
   const int slotid = tile_slot[tile];
   if (slotid < 0) return;
   const int lane = lane_id(), slot = lane / D, r = lane % D;
