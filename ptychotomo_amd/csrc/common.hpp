@@ -395,4 +395,15 @@ struct ProfScope {
   ~ProfScope();
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (linear id w runs on
+// XCD w % 8), each with its own L2.  xcd_contig renumbers linear ids so that
+// every XCD runs one contiguous range of logical ids (the first total % 8
+// XCDs one id more than the rest), in dispatch order: work items that read
+// overlapping regions share one L2.  A bijection on [0, total); results do
+// not depend on it.
+__device__ __forceinline__ int xcd_contig(int w, int total) {
+  const int q = total >> 3, r = total & 7, x = w & 7;
+  return x * q + min(x, r) + (w >> 3);
+}
+
 }  // namespace pt

@@ -52,20 +52,6 @@ constexpr int WAVE = 64;
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 
-// Workgroups are dispatched round-robin over the 8 XCDs (linear id w runs on
-// XCD w % 8), each with its own L2.  xcd_contig renumbers linear ids so that
-// every XCD runs one contiguous range of logical ids (the first total % 8
-// XCDs one id more than the rest), in dispatch order; a bijection on
-// [0, total), results do not depend on it.  Used by the overlap-add tiles
-// (adjacent tiles' extended regions overlap: 5.9 -> 5.6 ms per 256^3
-// iteration); the line-search and forward kernels measured slower with it
-// (line search 12.5 -> 15.7 ms per 256^3 iteration) and keep the hardware
-// order.
-__device__ __forceinline__ int xcd_contig(int w, int total) {
-  const int q = total >> 3, r = total & 7, x = w & 7;
-  return x * q + min(x, r) + (w >> 3);
-}
-
 // ------------------------------------------------------------------ transforms
 // Forward 2D transform of the frame rows held by the wave (v = row r of the
 // lane's patch) -> lane c holds far-field column c (in v).
@@ -583,6 +569,10 @@ __global__ __launch_bounds__(WAVE * tile_waves<D>()) void k_tile(
   float2* acc = reinterpret_cast<float2*>(lds_f);          // EB*EB
   const int wave = threadIdx.x / WAVE;
   float* T = lds_f + 2 * EB * EB + wave * G * D * (D + 1);  // this wave's buffer
+  // XCD-contiguous tile order: adjacent tiles' extended regions overlap (5.9
+  // -> 5.6 ms per 256^3 iteration; the line-search and forward kernels
+  // measured slower with it -- line search 12.5 -> 15.7 ms -- and keep the
+  // hardware order)
   const long tile = xcd_contig(blockIdx.x, gridDim.x);
   const int slotid = tile_slot[tile];
   if (slotid < 0) return;

@@ -209,7 +209,10 @@ __global__ __launch_bounds__(256) void k_gather_tile(float2* __restrict__ G,
                                                      int pz) {
   extern __shared__ float4 lds4[];  // [E][E] pairs of complex (z0, z0+1)
   const int twoN = 2 * N, E = kTile + 2 * M;
-  const int wi = blockIdx.x, z0 = 2 * blockIdx.y;
+  // XCD-contiguous order: adjacent tiles' staged regions overlap by 2M
+  const int lw = xcd_contig(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int zb = lw / gridDim.x;
+  const int wi = lw - zb * gridDim.x, z0 = 2 * zb;
   const bool two = z0 + 1 < pz;
   const int tile = wi_tile[wi];
   const int ty0 = (tile / ntile1) * kTile - N, tx0 = (tile % ntile1) * kTile - N;
@@ -370,7 +373,10 @@ __global__ __launch_bounds__(64) void k_scatter_own(float2* __restrict__ Bg,
   __shared__ float4 gs[kSE][kZS / 2 + 1];  // slice pairs (16-byte reads)
   __shared__ int eos[kSE];
   const int twoN = 2 * N;
-  const int wi = blockIdx.x, z0 = blockIdx.y * kZS;
+  // XCD-contiguous order: adjacent tiles share the samples of their entries
+  const int lw = xcd_contig(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int zb = lw / gridDim.x;
+  const int wi = lw - zb * gridDim.x, z0 = zb * kZS;
   const int lane = threadIdx.x, lx = lane & 7, ly = lane >> 3;
   const int tile = wi_tile[wi];
   const int gx = (tile % nt8) * kTS + lx, gy = (tile / nt8) * kTS + ly;
