@@ -126,16 +126,9 @@ __global__ __launch_bounds__(256) void k_row_fwd(float2* __restrict__ A, const f
 }
 
 // ------------------------------------------------------------------ forward column pass
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2.  xcd_pair renumbers linear workgroup ids so that logical tiles 2i and
-// 2i+1 run on one XCD back to back (they share the cache lines of the rows
-// they read): w -> (w & ~15) | (w & 7) << 1 | (w >> 3) & 1, a bijection on
-// every aligned group of 16 ids; ids of a ragged last group stay.
-__device__ __forceinline__ int xcd_pair(int w, int total) {
-  if (w >= (total & ~15)) return w;
-  return (w & ~15) | ((w & 7) << 1) | ((w >> 3) & 1);
-}
-
+// Column tiles run in XCD-contiguous order (xcd_contig): neighbouring tiles
+// read the same cache lines of the rows (8 columns = 64 bytes per row).
+//
 // C = 2^lc columns kx of one slice per workgroup: y[q] = A[z][q][kx] ->
 // centred 2N-point DFT along q -> B[z][ky][kx], as the plain N-point DFT
 // (even ky - N) and the twisted one (odd ky - N) run one after the other in
@@ -148,7 +141,7 @@ __global__ __launch_bounds__(256) void k_col_fwd(float2* __restrict__ Bg, const 
   extern __shared__ float2 lds[];
   const int N = plan.N, twoN = 2 * N, S = N + 1, C = 1 << lc;
   const int gx = (twoN + C - 1) >> lc;
-  const int L = xcd_pair(blockIdx.x + gx * blockIdx.y, gx * gridDim.y);
+  const int L = xcd_contig(blockIdx.x + gx * blockIdx.y, gx * gridDim.y);
   const int z = L / gx, k0 = (L - z * gx) << lc;
   float2* la = lds;
   float2* lb = la + C * S;
@@ -491,7 +484,7 @@ __global__ __launch_bounds__(256) void k_col_adj(float2* __restrict__ A, const f
   extern __shared__ float2 lds[];
   const int N = plan.N, twoN = 2 * N, S = N + 1, o = N / 6, C = 1 << lc;
   const int gx = (twoN + C - 1) >> lc;
-  const int L = xcd_pair(blockIdx.x + gx * blockIdx.y, gx * gridDim.y);
+  const int L = xcd_contig(blockIdx.x + gx * blockIdx.y, gx * gridDim.y);
   const int z = L / gx, k0 = (L - z * gx) << lc;
   float2* la = lds;
   float2* lb = la + C * S;
