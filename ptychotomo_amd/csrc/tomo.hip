@@ -186,11 +186,17 @@ __global__ __launch_bounds__(256) void k_col_fwd(float2* __restrict__ Bg, const 
 }
 
 // ------------------------------------------------------------------ gather
-// One workgroup per (32 x 32 grid tile bin chunk, slice pair): the tile plus
-// an M-point halo (periodic in 2N) is staged in LDS as [y][x][z-pair]; each
-// lane interpolates one polar sample of the bin with separable Gaussian
-// weights (kernels.cuh:97-126).  Output G[z][sample].
+// One workgroup per (32 x 32 grid tile bin chunk, kGatherZ slice pairs): the
+// tile plus an M-point halo (periodic in 2N) is staged in LDS as
+// [y][x][z-pair]; each lane interpolates polar samples of the bin with
+// separable Gaussian weights (kernels.cuh:97-126).  Output G[z][sample].
+// The slice pairs are pipelined: the loads of pair j+1's tile are in flight
+// (registers) while pair j is interpolated from LDS, and the lane's sample
+// positions are loaded once for all pairs, so the interpolation issues no
+// global load that would wait behind the prefetch.
 constexpr int kGatherStage = 9;  // staged points per thread: (kTile + 14)^2 <= 2304
+constexpr int kGatherZ = 4;      // slice pairs per workgroup
+constexpr int kGatherSpl = kWorkMax / 256;  // samples per lane
 __global__ __launch_bounds__(256) void k_gather_tile(float2* __restrict__ G,
                                                      const float2* __restrict__ Bg,
                                                      const float2* __restrict__ spos,
@@ -200,21 +206,37 @@ __global__ __launch_bounds__(256) void k_gather_tile(float2* __restrict__ G,
                                                      const int* __restrict__ bin_sample, int N,
                                                      int M, float beta, int ntile1, int ns,
                                                      int pz) {
-  extern __shared__ float4 lds4[];  // [E][E] pairs of complex (z0, z0+1)
+  extern __shared__ float4 lds4[];  // [E][E] pairs of complex (z, z+1)
   const int twoN = 2 * N, E = kTile + 2 * M;
   // XCD-contiguous order: adjacent tiles' staged regions overlap by 2M
   const int lw = xcd_contig(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
   const int zb = lw / gridDim.x;
-  const int wi = lw - zb * gridDim.x, z0 = 2 * zb;
-  const bool two = z0 + 1 < pz;
+  const int wi = lw - zb * gridDim.x;
+  const int npair = (pz + 1) / 2;
+  const int jp0 = zb * kGatherZ, jp1 = min(jp0 + kGatherZ, npair);
   const int tile = wi_tile[wi];
   const int ty0 = (tile / ntile1) * kTile - N, tx0 = (tile % ntile1) * kTile - N;
-  const float2* B0 = Bg + (long)z0 * twoN * twoN;
-  const float2* B1 = two ? B0 + (long)twoN * twoN : B0;
-  // every load of the staged tile first (one memory round trip), then the
-  // LDS stores: E*E <= 256*kGatherStage
-  {
-    float2 va[kGatherStage], vb[kGatherStage];
+  const int2 rg = wi_range[wi];
+  // this lane's samples (at most kGatherSpl: rg.y - rg.x <= kWorkMax)
+  int sid[kGatherSpl];
+  float2 sp[kGatherSpl];
+  int2 sf[kGatherSpl];
+#pragma unroll
+  for (int q = 0; q < kGatherSpl; ++q) {
+    const int bi = rg.x + threadIdx.x + q * 256;
+    sid[q] = bi < rg.y ? bin_sample[bi] : -1;
+  }
+#pragma unroll
+  for (int q = 0; q < kGatherSpl; ++q) {
+    sp[q] = sid[q] >= 0 ? spos[sid[q]] : make_float2(0.f, 0.f);
+    sf[q] = sid[q] >= 0 ? sfloor[sid[q]] : make_int2(0, 0);
+  }
+  float2 va[kGatherStage], vb[kGatherStage];
+  auto load_pair = [&](int jp) {
+    const int z0 = 2 * jp;
+    const bool two = z0 + 1 < pz;
+    const float2* B0 = Bg + (long)z0 * twoN * twoN;
+    const float2* B1 = two ? B0 + (long)twoN * twoN : B0;
 #pragma unroll
     for (int u = 0; u < kGatherStage; ++u) {
       const int i = threadIdx.x + u * 256;
@@ -225,49 +247,69 @@ __global__ __launch_bounds__(256) void k_gather_tile(float2* __restrict__ G,
       va[u] = in ? B0[off] : make_float2(0.f, 0.f);
       vb[u] = in && two ? B1[off] : make_float2(0.f, 0.f);
     }
+  };
+  load_pair(jp0);
+  const float inv2N = 1.0f / (float)twoN;
+  for (int jp = jp0; jp < jp1; ++jp) {
+    __syncthreads();  // the previous pair's interpolation has read the tile
 #pragma unroll
     for (int u = 0; u < kGatherStage; ++u) {
       const int i = threadIdx.x + u * 256;
       if (i < E * E) lds4[i] = make_float4(va[u].x, va[u].y, vb[u].x, vb[u].y);
     }
-  }
-  __syncthreads();
-  const int2 rg = wi_range[wi];
-  const float inv2N = 1.0f / (float)twoN;
-  for (int bi = rg.x + threadIdx.x; bi < rg.y; bi += blockDim.x) {
-    const int s = bin_sample[bi];
-    const float2 p = spos[s];
-    const int2 fl = sfloor[s];
-    float ex[15], ey[15];
+    __syncthreads();
+    if (jp + 1 < jp1) load_pair(jp + 1);
+    const int z0 = 2 * jp;
+    const bool two = z0 + 1 < pz;
+    // the lane's samples in turn from slot 0, rotated once per sample (four
+    // rotations restore the order for the next pair; registers, no dynamic
+    // indexing)
+#pragma unroll 1
+    for (int q = 0; q < kGatherSpl; ++q) {
+      const int s = sid[0];
+      const float2 p = sp[0];
+      const int2 fl = sf[0];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) {
-      const float dx = (float)(fl.x - M + k) * inv2N - p.x;
-      const float dy = (float)(fl.y - M + k) * inv2N - p.y;
-      ex[k] = (k <= 2 * M) ? __expf(-beta * dx * dx) : 0.f;
-      ey[k] = (k <= 2 * M) ? __expf(-beta * dy * dy) : 0.f;
-    }
-    const int row = fl.y - ty0, col = fl.x - tx0;  // footprint origin in the staged tile
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int k1 = 0; k1 <= 2 * M; ++k1) {
-      const float4* rp = lds4 + (row + k1) * E + col;
-      float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-      for (int k0 = 0; k0 < 15; ++k0) {
-        if (k0 <= 2 * M) {
-          const float4 v = rp[k0];
-          r.x += ex[k0] * v.x;
-          r.y += ex[k0] * v.y;
-          r.z += ex[k0] * v.z;
-          r.w += ex[k0] * v.w;
-        }
+      for (int k = 0; k < kGatherSpl - 1; ++k) {
+        sid[k] = sid[k + 1];
+        sp[k] = sp[k + 1];
+        sf[k] = sf[k + 1];
       }
-      acc.x += ey[k1] * r.x;
-      acc.y += ey[k1] * r.y;
-      acc.z += ey[k1] * r.z;
-      acc.w += ey[k1] * r.w;
+      sid[kGatherSpl - 1] = s;
+      sp[kGatherSpl - 1] = p;
+      sf[kGatherSpl - 1] = fl;
+      if (s < 0) continue;
+      float ex[15], ey[15];
+#pragma unroll
+      for (int k = 0; k < 15; ++k) {
+        const float dx = (float)(fl.x - M + k) * inv2N - p.x;
+        const float dy = (float)(fl.y - M + k) * inv2N - p.y;
+        ex[k] = (k <= 2 * M) ? __expf(-beta * dx * dx) : 0.f;
+        ey[k] = (k <= 2 * M) ? __expf(-beta * dy * dy) : 0.f;
+      }
+      const int row = fl.y - ty0, col = fl.x - tx0;  // footprint origin in the staged tile
+      float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int k1 = 0; k1 <= 2 * M; ++k1) {
+        const float4* rp = lds4 + (row + k1) * E + col;
+        float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k0 = 0; k0 < 15; ++k0) {
+          if (k0 <= 2 * M) {
+            const float4 v = rp[k0];
+            r.x += ex[k0] * v.x;
+            r.y += ex[k0] * v.y;
+            r.z += ex[k0] * v.z;
+            r.w += ex[k0] * v.w;
+          }
+        }
+        acc.x += ey[k1] * r.x;
+        acc.y += ey[k1] * r.y;
+        acc.z += ey[k1] * r.z;
+        acc.w += ey[k1] * r.w;
+      }
+      G[(long)z0 * ns + s] = make_float2(acc.x, acc.y);
+      if (two) G[(long)(z0 + 1) * ns + s] = make_float2(acc.z, acc.w);
     }
-    G[(long)z0 * ns + s] = make_float2(acc.x, acc.y);
-    if (two) G[(long)(z0 + 1) * ns + s] = make_float2(acc.z, acc.w);
   }
 }
 
@@ -623,7 +665,7 @@ int run_fwd(pt_tomo* h, float2* g, const float2* f, float scale, hipStream_t st)
   {
     const float beta = (float)(kPiRef * kPiRef / (double)h->mu);
     const int E = kTile + 2 * h->M;
-    dim3 grid(h->nwi, (h->pz + 1) / 2);
+    dim3 grid(h->nwi, ((h->pz + 1) / 2 + kGatherZ - 1) / kGatherZ);
     ProfScope prof("tomo_gather", st);
     hipLaunchKernelGGL(k_gather_tile, grid, dim3(256), sizeof(float4) * E * E, st, h->Q, h->Bg,
                        h->spos, h->sfloor, h->wi_tile, h->wi_range, h->bin_sample, N, h->M, beta,
