@@ -199,8 +199,18 @@ __global__ __launch_bounds__(WAVE) void k_fwd(float2* __restrict__ g, const floa
                                               const float2* __restrict__ shift, int pt, int nz, int n,
                                               int S, int P, int ox, int oy, float cs) {
   constexpr int G = WAVE / D;
-  __shared__ float2 T[G * D * (D + 1)];
+  // frames up to 32 x 32 transpose one float component at a time (half the
+  // LDS of a complex buffer; at 64 x 64 that form spills registers); frames
+  // up to 16 x 16 read the probe from LDS (a lane's probe column is P-strided
+  // in memory: one cache line per lane and row)
+  __shared__ float Tf[D <= 32 ? G * D * (D + 1) : 1];
+  __shared__ float2 Tc[D <= 32 ? 1 : G * D * (D + 1)];
+  __shared__ float2 prl[D <= 16 ? D * D : 1];
   const int lane = lane_id(), slot = lane / D, r = lane % D;
+  if constexpr (D <= 16) {
+    for (int i = lane; i < P * P; i += WAVE) prl[i] = prb[i];
+    __syncthreads();
+  }
   const long npatch = (long)pt * S;
   for (long grp = blockIdx.x; grp * G < npatch; grp += gridDim.x) {
     const long p = grp * G + slot;
@@ -208,8 +218,14 @@ __global__ __launch_bounds__(WAVE) void k_fwd(float2* __restrict__ g, const floa
     int2 ps = active ? pos[p] : make_int2(-1, -1);
     const int t = active ? (int)(p / S) : 0;
     float2 v[D];
-    load_frame_row<D>(v, f, prb, ps, t, r, nz, n, P, ox, oy, cs, active);
-    frame_fwd<D>(v, T, slot, r);
+    if constexpr (D <= 16) {
+      load_rows<D>(v, f, ps, t, r, nz, n, P, ox, oy, active);
+      apply_probe<D>(v, prl, r, P, ox, oy, cs);
+    } else {
+      load_frame_row<D>(v, f, prb, ps, t, r, nz, n, P, ox, oy, cs, active);
+    }
+    if constexpr (D <= 32) frame_fwd_w<D>(v, Tf, slot, r);
+    else frame_fwd<D>(v, Tc, slot, r);
     if (active) {
       float2 sh = shift[p];
       float2* gp = g + p * D * D + r;
