@@ -68,20 +68,6 @@ __device__ __forceinline__ void frame_fwd(float2 (&v)[D], float2* T, int slot, i
   reg_fft<D, -1>(v);
 }
 
-// Inverse of frame_fwd's layout: lane c holds column c -> lane r holds row r.
-template <int D>
-__device__ __forceinline__ void frame_inv(float2 (&v)[D], float2* T, int slot, int r) {
-  reg_fft<D, 1>(v);
-  float2* Ts = T + slot * D * (D + 1);
-#pragma unroll
-  for (int k = 0; k < D; ++k) Ts[k * (D + 1) + r] = v[k];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < D; ++k) v[k] = Ts[r * (D + 1) + k];
-  __syncthreads();
-  reg_fft<D, 1>(v);
-}
-
 // Wave-local LDS exchange: orders the wave's LDS accesses (LDS operations of
 // one wavefront complete in issue order) without a workgroup barrier.
 __device__ __forceinline__ void wave_sync() {
