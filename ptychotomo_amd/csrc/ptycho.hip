@@ -97,7 +97,7 @@ __device__ __forceinline__ void transpose_w(float2 (&v)[D], float* Ts, int r) {
   }
   wave_sync();
 }
-// frame_fwd / frame_inv with the wave-local float buffer (T: this wave's
+// Forward / inverse frame transforms with the wave-local float buffer (T: this wave's
 // G*D*(D+1) floats)
 template <int D>
 __device__ __forceinline__ void frame_fwd_w(float2 (&v)[D], float* T, int slot, int r) {
