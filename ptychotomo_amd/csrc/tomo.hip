@@ -197,6 +197,7 @@ __global__ __launch_bounds__(256) void k_col_fwd(float2* __restrict__ Bg, const 
 constexpr int kGatherStage = 9;  // staged points per thread: (kTile + 14)^2 <= 2304
 constexpr int kGatherZ = 4;      // slice pairs per workgroup
 constexpr int kGatherSpl = kWorkMax / 256;  // samples per lane
+static_assert(kGatherSpl * 256 == kWorkMax, "a gather work item's samples fill whole lanes");
 __global__ __launch_bounds__(256) void k_gather_tile(float2* __restrict__ G,
                                                      const float2* __restrict__ Bg,
                                                      const float2* __restrict__ spos,
