@@ -232,12 +232,15 @@ __device__ __forceinline__ float log_abs(float2 a, float s) {
 }
 
 // log(1 + x): six-term series for |x| < 1/16 (truncation below 4e-10
-// relative), the hardware log of 1 + x otherwise (1 + x then exact to 6e-8
-// relative of a log of at least 0.06)
+// relative), otherwise the hardware base-2 log of 1 + x times ln 2 (1 + x
+// then exact to 6e-8 relative of a log of at least 0.06; it is 0 or at least
+// 2^-24, never subnormal, so the library log's subnormal scaling and split
+// ln 2 product -- 11 more instructions -- buy nothing here; the single
+// product adds at most one rounding)
 __device__ __forceinline__ float log1p_step(float x) {
   if (fabsf(x) < 0.0625f)
     return x * (1.f + x * (-0.5f + x * (0.33333334f + x * (-0.25f + x * (0.2f + x * -0.16666667f)))));
-  return __logf(1.f + x);
+  return __builtin_amdgcn_logf(1.f + x) * 0.693147181f;
 }
 
 // Line-search decision quantity of one lane's far-field column: the change
