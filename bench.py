@@ -230,7 +230,7 @@ def main():
                     help="size of the plain-vs-warm comparison (plain ADMM does not converge "
                          "on the 256^3 workload)")
     ap.add_argument("--tol-iters", type=int, default=100)
-    ap.add_argument("--tol-seeds", type=int, default=3, help="held-out phantoms compared")
+    ap.add_argument("--tol-seeds", type=int, default=7, help="held-out phantoms compared")
     ap.add_argument("--no-warm", action="store_true", help="skip the plain-vs-warm comparison")
     ap.add_argument("--no-operators", action="store_true",
                     help="skip the whole-volume operator rates (profiled runs of the iteration)")
